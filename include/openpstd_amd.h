@@ -1,0 +1,231 @@
+/*
+ * openpstd_amd — MI355X-native PSTD update loop behind openPSTD's KernelInterface.
+ *
+ * This header is the drop-in boundary: a plain C ABI (no C++ or torch types).
+ *
+ *  1. Configuration types mirroring openPSTD's PSTDSettings / DomainConf /
+ *     PSTDConfiguration (kernel/KernelInterface.h:44-241).
+ *  2. The device solver context (pstd_ctx_*): what openPSTD's GPU solver slot,
+ *     GPUSingleThreadSolver::compute_propagation (kernel/Solver.cpp:153-156) and
+ *     GPUMultiThreadSolver::compute_propagation (kernel/Solver.cpp:157-160),
+ *     binds to advance a scene on the GPU.  The scene is handed over as a flat
+ *     plan (domains, line segments, receivers) built from openPSTD's Scene.
+ *  3. The kernel facade (pstd_kernel_*): the C form of
+ *     KernelInterface::{initialize_kernel, run, get_metadata}
+ *     (kernel/KernelInterface.h:332-351) as implemented by
+ *     PSTDKernel(bool GPU, bool MCPU) (kernel/PSTDKernel.h:52-150), for callers
+ *     that are not C++ (the Python package binds this through ctypes).
+ *
+ * All functions return PSTD_OK (0) on success and a negative status on
+ * failure; the message is available from pstd_ctx_last_error /
+ * pstd_kernel_last_error.  No exception crosses this boundary.
+ */
+#ifndef OPENPSTD_AMD_H
+#define OPENPSTD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define PSTD_OK 0
+#define PSTD_ERR_INVALID (-1)        /* invalid argument or unsupported scene */
+#define PSTD_ERR_NOT_CONFIGURED (-2) /* PSTDKernelNotConfiguredException (kernel/KernelInterface.h:353-356) */
+#define PSTD_ERR_DEVICE (-3)         /* HIP runtime error */
+#define PSTD_ERR_UNSUPPORTED (-4)    /* solver flavour not provided by this library */
+#define PSTD_ERR_ALLOC (-5)          /* device or host allocation failed */
+
+/* -------------------------------------------------------- configuration */
+/* Mirrors PSTDSettings (kernel/KernelInterface.h:44-158); same fields, same
+ * units (SI), same serialisation order (kernel/KernelInterface.h:80-97). */
+typedef struct pstd_settings {
+    float calctime;         /* simulation length [s]; steps = (int)(calctime/dt) */
+    float c1;               /* speed of sound [m/s] */
+    float ampmax;           /* PML attenuation amplitude */
+    float freqmax;          /* maximum frequency [Hz] (not used by the loop) */
+    float rho;              /* density of air [kg/m^3] */
+    float patcherror;       /* patch error [dB]; window = round(round((pe*0.7-17)/2)*2) */
+    float tfactRK;          /* CFL factor; dt = tfactRK*dx/c1 */
+    float grid_spacing;     /* dx [m] */
+    float band_width;       /* Gaussian source bandwidth beta */
+    float wave_length;      /* only used by spectral interpolation (disabled in openPSTD) */
+    float rk_coefficients[6];
+    int32_t spectral_interpolation;
+    int32_t pml_cells;
+    int32_t save_nth;
+} pstd_settings;
+
+/* DomainConfEdge (kernel/KernelInterface.h:163-176) */
+typedef struct pstd_edge {
+    float absorption; /* in [0, 1] */
+    int32_t lr;       /* locally reacting flag */
+} pstd_edge;
+
+/* DomainConf (kernel/KernelInterface.h:181-206); world metres. */
+typedef struct pstd_domain_conf {
+    float top_left[2];
+    float size[2];
+    pstd_edge T, L, B, R;
+} pstd_domain_conf;
+
+/* Receiver sampling rule.  openPSTD's nearest-cell receiver reads
+ * p0(rel.x, rel.y) with p0 stored (row = y, col = x)
+ * (kernel/core/Receiver.cpp:80-83), i.e. the transposed cell.
+ * PSTD_RECEIVER_PHYSICAL (default) samples p0[rel.y][rel.x];
+ * PSTD_RECEIVER_REFERENCE samples row rel.x, column rel.y as openPSTD does and
+ * rejects receivers for which that cell is outside the domain (openPSTD's
+ * Eigen bounds assertion). */
+#define PSTD_RECEIVER_PHYSICAL 0
+#define PSTD_RECEIVER_REFERENCE 1
+
+/* PSTDConfiguration (kernel/KernelInterface.h:214-241). */
+typedef struct pstd_config {
+    pstd_settings settings;
+    int32_t n_speakers;
+    const float *speakers;   /* n_speakers x {x, y, z} world metres */
+    int32_t n_receivers;
+    const float *receivers;  /* n_receivers x {x, y, z} world metres */
+    int32_t n_domains;
+    const pstd_domain_conf *domains;
+    int32_t receiver_mode;   /* PSTD_RECEIVER_* */
+} pstd_config;
+
+/* ------------------------------------------------------- device solver */
+/* Field identifiers for pstd_ctx_set_field / pstd_ctx_get_field; they name
+ * FieldValues members (kernel/core/Domain.h:57-63). */
+#define PSTD_FIELD_P0 0
+#define PSTD_FIELD_PX 1
+#define PSTD_FIELD_PY 2
+#define PSTD_FIELD_VX 3
+#define PSTD_FIELD_VY 4
+
+/* One rectangular domain of the scene (kernel/core/Domain.h:103-153).
+ * Arrays are row-major (row = y, column = x), the frame order of
+ * Solver::get_pressure_vector (kernel/Solver.cpp:208-217):
+ * p0/px/py ny x nx, vx ny x (nx+1), vy (ny+1) x nx. */
+typedef struct pstd_plan_domain {
+    int32_t nx, ny;
+    float rho;               /* Domain::rho */
+    int32_t is_pml;          /* receives PML attenuation after every step */
+    int32_t is_rigid;        /* Domain::is_rigid(): never updated */
+    const float *pml_px;     /* ny x nx PML factors, or NULL when !is_pml */
+    const float *pml_py;     /* ny x nx */
+    const float *pml_vx;     /* ny x (nx+1) */
+    const float *pml_vy;     /* (ny+1) x nx */
+} pstd_plan_domain;
+
+/* A run of lines of one domain that share neighbours and stencil: one
+ * spatderp3 call of Domain::calc (kernel/core/Domain.cpp:149-370).
+ * direction 0 = X (lines are rows), 1 = Y (lines are columns);
+ * field 0 = pressure (derivative of p0 -> updates vx/vy),
+ *       1 = velocity (derivative of vx/vy -> updates px/py). */
+typedef struct pstd_plan_segment {
+    int32_t domain;
+    int32_t direction;
+    int32_t field;
+    int32_t line_begin, line_end;   /* local line range [begin, end) */
+    int32_t side1, side2;           /* neighbour domains (left/bottom, right/top) or -1 = zeros */
+    int32_t side1_offset, side2_offset; /* neighbour-local line = line + offset */
+    int32_t wlen;                   /* window half length w (Domain::calc halves it for short domains) */
+    int32_t table_len;              /* length of the wavenumber table derfact is read from */
+    float c_side1, c_main1, c_side2, c_main2; /* get_rho_array stencil coefficients */
+} pstd_plan_segment;
+
+typedef struct pstd_plan_receiver {
+    int32_t domain;
+    int32_t index;                  /* row-major index into the domain's p0 */
+} pstd_plan_receiver;
+
+typedef struct pstd_plan_desc {
+    int32_t n_domains;
+    const pstd_plan_domain *domains;
+    int32_t n_segments;
+    const pstd_plan_segment *segments;
+    int32_t n_receivers;
+    const pstd_plan_receiver *receivers;
+    float dx;                       /* grid spacing */
+    float dt;                       /* PSTDSettings::GetTimeStep() */
+    float c1;                       /* speed of sound */
+    float patch_error;              /* window shape */
+    float rk_alpha[6];              /* low-storage RK coefficients */
+} pstd_plan_desc;
+
+typedef struct pstd_ctx pstd_ctx;
+
+/* Creates a solver context on HIP device `device`. */
+int pstd_ctx_create(int device, pstd_ctx **out);
+/* Uploads the scene plan: allocates device state (zeroed), builds the device
+ * plan cache (derivative filters, windows, twiddles, line tiles). */
+int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan);
+/* Host <-> device copies of one field of one domain (FieldValues::current). */
+int pstd_ctx_set_field(pstd_ctx *ctx, int domain, int field, const float *host, size_t count);
+int pstd_ctx_get_field(pstd_ctx *ctx, int domain, int field, float *host, size_t count);
+/* Advances `steps` time steps (compute_timestep, kernel/Solver.cpp:71-93):
+ * 6 RK stages, PML attenuation, receiver sampling.  When `samples` is not
+ * NULL it receives steps x n_receivers receiver values (one per step, in
+ * receiver order).  Blocks until the samples are on the host. */
+int pstd_ctx_advance(pstd_ctx *ctx, int steps, float *samples);
+/* Like pstd_ctx_advance but does not synchronise and keeps the samples on the
+ * device (bench and pipelined callers); pstd_ctx_synchronize waits. */
+int pstd_ctx_advance_async(pstd_ctx *ctx, int steps);
+int pstd_ctx_synchronize(pstd_ctx *ctx);
+/* Current p0 of one domain, row-major (Solver::get_pressure_vector). */
+int pstd_ctx_read_frame(pstd_ctx *ctx, int domain, float *out, size_t count);
+/* Number of time steps advanced since upload. */
+int pstd_ctx_steps_done(const pstd_ctx *ctx);
+const char *pstd_ctx_last_error(const pstd_ctx *ctx);
+void pstd_ctx_destroy(pstd_ctx *ctx);
+
+/* ------------------------------------------------------- kernel facade */
+/* KernelCallback (kernel/KernelInterface.h:243-304) as C function pointers.
+ * write_frame: WriteFrame(frame, domain, data) — fresh row-major p0 per call.
+ * write_sample: WriteSample(startSample, receiver, data).
+ * log: level 0 Fatal, 1 Error, 2 Warning, 3 Info, 4 Debug. */
+typedef struct pstd_callbacks {
+    void *user;
+    void (*write_frame)(void *user, int frame, int domain, const float *data, size_t n);
+    void (*write_sample)(void *user, int start_sample, int receiver, const float *data, size_t n);
+    void (*log)(void *user, int level, const char *message);
+} pstd_callbacks;
+
+typedef struct pstd_kernel pstd_kernel;
+
+/* PSTDKernel(bool GPU, bool MCPU) (kernel/PSTDKernel.cpp:44-47).  This
+ * library provides the GPU solver slot: GPU=0 is PSTD_ERR_UNSUPPORTED at run. */
+int pstd_kernel_create(int gpu, int mcpu, pstd_kernel **out);
+/* initialize_kernel (kernel/PSTDKernel.cpp:49-59). */
+int pstd_kernel_initialize(pstd_kernel *k, const pstd_config *conf, const pstd_callbacks *log);
+/* run (kernel/PSTDKernel.cpp:121-148): blocking; callbacks on the caller's thread. */
+int pstd_kernel_run(pstd_kernel *k, const pstd_callbacks *cb);
+/* get_metadata (kernel/PSTDKernel.cpp:154-168): sizes of every domain
+ * (PML included) as x,y,z triples, and the frame count. */
+int pstd_kernel_get_metadata(pstd_kernel *k, int32_t *n_domains, int32_t *sizes_xyz, int32_t max_domains,
+                             int32_t *framecount);
+/* Scene inspection (the test-only Scene access of PSTDKernel::get_scene,
+ * kernel/PSTDKernel.cpp:150-152).  info receives 16 ints:
+ * id, tl.x, tl.y, size.x, size.y, is_pml, is_secondary_pml, is_rigid, local,
+ * n_left, n_right, n_top, n_bottom, update_x, update_y, pml_for_count;
+ * finfo receives alpha, impedance, rho. */
+int pstd_kernel_domain_info(pstd_kernel *k, int index, int32_t *info, float *finfo);
+/* Current field of domain `index` (after initialisation, or after run). */
+int pstd_kernel_read_field(pstd_kernel *k, int index, int field, float *out, size_t count);
+/* PML factor array (0 px, 1 py, 2 vx, 3 vy) of domain `index`. */
+int pstd_kernel_read_pml(pstd_kernel *k, int index, int which, float *out, size_t count);
+/* Builds the device context of the configured scene (plan uploaded, initial
+ * fields set, frame 0 next) without running it, and returns it in *ctx (owned
+ * by the kernel; valid until the next prepare/run/destroy).  For callers that
+ * drive the time loop themselves (benchmarks, stepwise inspection). */
+int pstd_kernel_prepare(pstd_kernel *k, pstd_ctx **ctx);
+/* Plan the kernel hands to the device (segments etc.) — for tests. */
+int pstd_kernel_segment_count(pstd_kernel *k);
+int pstd_kernel_get_segment(pstd_kernel *k, int i, pstd_plan_segment *out);
+const char *pstd_kernel_last_error(const pstd_kernel *k);
+void pstd_kernel_destroy(pstd_kernel *k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPENPSTD_AMD_H */

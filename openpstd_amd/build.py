@@ -1,0 +1,75 @@
+"""Builds libopenpstd_amd.so (HIP device code for gfx950 + host C++) in-tree.
+
+    python -m openpstd_amd.build
+
+Objects go to openpstd_amd/_build/, the library to openpstd_amd/_lib/.
+"""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIBDIR = os.path.join(HERE, "_lib")
+LIB = os.path.join(LIBDIR, "libopenpstd_amd.so")
+ARCH = os.environ.get("OPENPSTD_AMD_ARCH", "gfx950")
+
+HIP_SOURCES = ["device/pstd_device.hip"]
+CXX_SOURCES = ["host/scene_model.cpp", "host/pstd_kernel.cpp", "host/capi.cpp"]
+HEADERS = ["device/fft_line.h", "host/kernel_api.h", "host/scene_model.h", "host/pstd_kernel.h"]
+
+# Packed FP32 instructions are kept out of the device code.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+             "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
+
+
+def _hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + r.stdout)
+    return r.stdout
+
+
+def build(verbose=False):
+    """Compiles every source (always) and links the shared library."""
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    hipcc = _hipcc()
+    inc = ["-I" + os.path.join(HERE, "..", "include")]
+    jobs = []
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        jobs.append([hipcc] + HIP_FLAGS + inc + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    cxx = shutil.which("g++") or "g++"
+    for src in CXX_SOURCES:
+        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        jobs.append([cxx] + CXX_FLAGS + inc + ["-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+                                               "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        outs = list(ex.map(_run, jobs))
+    if verbose:
+        for o in outs:
+            if o.strip():
+                print(o)
+    tmp = LIB + ".tmp"
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

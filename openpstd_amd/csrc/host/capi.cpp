@@ -1,0 +1,230 @@
+// C ABI of the kernel facade (pstd_kernel_* in include/openpstd_amd.h).
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "pstd_kernel.h"
+
+using namespace OpenPSTD::Kernel;
+
+struct pstd_kernel {
+    std::unique_ptr<PSTDKernel> k;
+    std::string err;
+};
+
+namespace {
+
+class CLog : public KernelCallback {
+public:
+    explicit CLog(const pstd_callbacks *c) {
+        if (c) cb = *c;
+    }
+    void Fatal(std::string m) override { log(0, m); }
+    void Error(std::string m) override { log(1, m); }
+    void Warning(std::string m) override { log(2, m); }
+    void Info(std::string m) override { log(3, m); }
+    void Debug(std::string m) override { log(4, m); }
+    void WriteFrame(int frame, int domain, PSTD_FRAME_PTR data) override {
+        if (cb.write_frame) cb.write_frame(cb.user, frame, domain, data->data(), data->size());
+    }
+    void WriteSample(int start, int receiver, std::vector<float> data) override {
+        if (cb.write_sample) cb.write_sample(cb.user, start, receiver, data.data(), data.size());
+    }
+
+private:
+    void log(int level, const std::string &m) {
+        if (cb.log) cb.log(cb.user, level, m.c_str());
+    }
+    pstd_callbacks cb{};
+};
+
+std::shared_ptr<PSTDConfiguration> to_conf(const pstd_config *c) {
+    auto conf = std::make_shared<PSTDConfiguration>();
+    const pstd_settings &s = c->settings;
+    PSTDSettings &st = conf->Settings;
+    st.SetRenderTime(s.calctime);
+    st.SetSoundSpeed(s.c1);
+    st.SetAttenuationOfPMLCells(s.ampmax);
+    st.SetMaxFrequency(s.freqmax);
+    st.SetDensityOfAir(s.rho);
+    st.SetPatchError(s.patcherror);
+    st.SetFactRK(s.tfactRK);
+    st.SetGridSpacing(s.grid_spacing);
+    st.SetBandWidth(s.band_width);
+    st.SetWaveLength(s.wave_length);
+    st.SetRKCoefficients(std::vector<float>(s.rk_coefficients, s.rk_coefficients + 6));
+    st.SetSpectralInterpolation(s.spectral_interpolation != 0);
+    st.SetPMLCells(s.pml_cells);
+    st.SetSaveNth(s.save_nth);
+    for (int i = 0; i < c->n_speakers; ++i)
+        conf->Speakers.emplace_back(c->speakers[3 * i], c->speakers[3 * i + 1], c->speakers[3 * i + 2]);
+    for (int i = 0; i < c->n_receivers; ++i)
+        conf->Receivers.emplace_back(c->receivers[3 * i], c->receivers[3 * i + 1], c->receivers[3 * i + 2]);
+    for (int i = 0; i < c->n_domains; ++i) {
+        const pstd_domain_conf &dc = c->domains[i];
+        DomainConf d;
+        d.TopLeft = QVector2D(dc.top_left[0], dc.top_left[1]);
+        d.Size = QVector2D(dc.size[0], dc.size[1]);
+        d.T = {dc.T.absorption, dc.T.lr != 0};
+        d.L = {dc.L.absorption, dc.L.lr != 0};
+        d.B = {dc.B.absorption, dc.B.lr != 0};
+        d.R = {dc.R.absorption, dc.R.lr != 0};
+        conf->Domains.push_back(d);
+    }
+    conf->ReceiverMode = c->receiver_mode;
+    return conf;
+}
+
+template <class F>
+int guarded(pstd_kernel *k, F &&f) {
+    if (!k) return PSTD_ERR_INVALID;
+    try {
+        k->err.clear();
+        return f();
+    } catch (const PSTDKernelNotConfiguredException &e) {
+        k->err = e.what();
+        return PSTD_ERR_NOT_CONFIGURED;
+    } catch (const PSTDSolverError &e) {
+        k->err = e.what();
+        return e.status < 0 ? e.status : PSTD_ERR_INVALID;
+    } catch (const std::bad_alloc &) {
+        k->err = "out of memory";
+        return PSTD_ERR_ALLOC;
+    } catch (const std::exception &e) {
+        k->err = e.what();
+        return PSTD_ERR_INVALID;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pstd_kernel_create(int gpu, int mcpu, pstd_kernel **out) {
+    if (!out) return PSTD_ERR_INVALID;
+    *out = new pstd_kernel();
+    (*out)->k.reset(new PSTDKernel(gpu != 0, mcpu != 0));
+    return PSTD_OK;
+}
+
+void pstd_kernel_destroy(pstd_kernel *k) { delete k; }
+
+const char *pstd_kernel_last_error(const pstd_kernel *k) { return k ? k->err.c_str() : "null kernel"; }
+
+int pstd_kernel_initialize(pstd_kernel *k, const pstd_config *conf, const pstd_callbacks *log) {
+    return guarded(k, [&] {
+        if (!conf) return PSTD_ERR_INVALID;
+        k->k->initialize_kernel(to_conf(conf), std::make_shared<CLog>(log));
+        return PSTD_OK;
+    });
+}
+
+int pstd_kernel_run(pstd_kernel *k, const pstd_callbacks *cb) {
+    return guarded(k, [&] {
+        k->k->run(std::make_shared<CLog>(cb));
+        return PSTD_OK;
+    });
+}
+
+int pstd_kernel_get_metadata(pstd_kernel *k, int32_t *n_domains, int32_t *sizes_xyz, int32_t max_domains,
+                             int32_t *framecount) {
+    return guarded(k, [&] {
+        SimulationMetadata m = k->k->get_metadata();
+        if (n_domains) *n_domains = (int32_t)m.DomainMetadata.size();
+        if (framecount) *framecount = m.Framecount;
+        if (sizes_xyz)
+            for (int i = 0; i < (int)m.DomainMetadata.size() && i < max_domains; ++i)
+                for (int j = 0; j < 3; ++j) sizes_xyz[3 * i + j] = m.DomainMetadata[i][j];
+        return PSTD_OK;
+    });
+}
+
+int pstd_kernel_domain_info(pstd_kernel *k, int index, int32_t *info, float *finfo) {
+    return guarded(k, [&] {
+        if (!k->k->configured()) throw PSTDKernelNotConfiguredException();
+        const pstdamd::SceneModel &s = k->k->get_scene();
+        if (index < 0 || index >= s.n_domains()) return PSTD_ERR_INVALID;
+        const pstdamd::DomainModel &d = s.domain(index);
+        int32_t v[16] = {d.id, d.tlx, d.tly, d.nx, d.ny, d.is_pml, d.is_secondary, d.rigid(), d.local,
+                         (int)d.nb[pstdamd::kLeft].size(), (int)d.nb[pstdamd::kRight].size(),
+                         (int)d.nb[pstdamd::kTop].size(), (int)d.nb[pstdamd::kBottom].size(), d.update[0],
+                         d.update[1], (int)d.pml_for.size()};
+        if (info) std::memcpy(info, v, sizeof v);
+        if (finfo) {
+            finfo[0] = d.alpha;
+            finfo[1] = d.impedance;
+            finfo[2] = d.rho;
+        }
+        return PSTD_OK;
+    });
+}
+
+int pstd_kernel_read_field(pstd_kernel *k, int index, int field, float *out, size_t count) {
+    return guarded(k, [&] {
+        if (!k->k->configured()) throw PSTDKernelNotConfiguredException();
+        const pstdamd::SceneModel &s = k->k->get_scene();
+        if (index < 0 || index >= s.n_domains() || !out) return PSTD_ERR_INVALID;
+        const pstdamd::DomainModel &d = s.domain(index);
+        size_t n = field == PSTD_FIELD_VX ? (size_t)d.ny * (d.nx + 1)
+                   : field == PSTD_FIELD_VY ? (size_t)(d.ny + 1) * d.nx
+                                            : (size_t)d.nx * d.ny;
+        if (count != n) {
+            k->err = "field size mismatch";
+            return PSTD_ERR_INVALID;
+        }
+        const GPUSolver *sv = k->k->last_solver();
+        if (sv && sv->context()) {
+            int rc = pstd_ctx_get_field(sv->context(), index, field, out, count);
+            if (rc) k->err = pstd_ctx_last_error(sv->context());
+            return rc;
+        }
+        switch (field) {
+            case PSTD_FIELD_P0: std::memcpy(out, d.p0.data(), n * sizeof(float)); break;
+            case PSTD_FIELD_PX: std::memcpy(out, d.px0.data(), n * sizeof(float)); break;
+            case PSTD_FIELD_PY: std::memcpy(out, d.py0.data(), n * sizeof(float)); break;
+            case PSTD_FIELD_VX:
+            case PSTD_FIELD_VY: std::memset(out, 0, n * sizeof(float)); break;
+            default: return PSTD_ERR_INVALID;
+        }
+        return PSTD_OK;
+    });
+}
+
+int pstd_kernel_read_pml(pstd_kernel *k, int index, int which, float *out, size_t count) {
+    return guarded(k, [&] {
+        if (!k->k->configured()) throw PSTDKernelNotConfiguredException();
+        const pstdamd::SceneModel &s = k->k->get_scene();
+        if (index < 0 || index >= s.n_domains() || !out || which < 0 || which > 3) return PSTD_ERR_INVALID;
+        const pstdamd::DomainModel &d = s.domain(index);
+        const std::vector<float> *v = which == 0 ? &d.pml_px : which == 1 ? &d.pml_py : which == 2 ? &d.pml_vx : &d.pml_vy;
+        if (v->size() != count) {
+            k->err = "PML array size mismatch (or not a PML domain)";
+            return PSTD_ERR_INVALID;
+        }
+        std::memcpy(out, v->data(), count * sizeof(float));
+        return PSTD_OK;
+    });
+}
+
+int pstd_kernel_prepare(pstd_kernel *k, pstd_ctx **ctx) {
+    return guarded(k, [&] {
+        if (!ctx) return PSTD_ERR_INVALID;
+        *ctx = k->k->prepare_solver()->context();
+        return PSTD_OK;
+    });
+}
+
+int pstd_kernel_segment_count(pstd_kernel *k) {
+    if (!k || !k->k->configured()) return -1;
+    return (int)k->k->get_scene().segments().size();
+}
+
+int pstd_kernel_get_segment(pstd_kernel *k, int i, pstd_plan_segment *out) {
+    if (!k || !k->k->configured() || !out) return PSTD_ERR_INVALID;
+    const auto &segs = k->k->get_scene().segments();
+    if (i < 0 || i >= (int)segs.size()) return PSTD_ERR_INVALID;
+    *out = segs[i];
+    return PSTD_OK;
+}
+
+}  // extern "C"

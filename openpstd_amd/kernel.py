@@ -1,0 +1,246 @@
+"""openPSTD's kernel API (kernel/KernelInterface.h:243-356, kernel/PSTDKernel.h:52-150)
+over the native library.
+
+    kernel = PSTDKernel(GPU=True, MCPU=False)
+    kernel.initialize_kernel(conf, KernelCallbackLog())
+    kernel.run(callback)            # WriteFrame / WriteSample / Info ... on this thread
+    meta = kernel.get_metadata()
+
+run() drives the MI355X solver; there is no CPU path in this package.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi_types as T
+from ._lib import PSTDError, lib
+
+
+class PSTDKernelNotConfiguredException(RuntimeError):
+    """kernel/KernelInterface.h:353-356."""
+
+    def __init__(self):
+        super().__init__("Kernel is not yet configured")
+
+
+class KernelCallbackLog:
+    """kernel/KernelInterface.h:243-271: non-blocking log sink (no-op defaults)."""
+
+    def Fatal(self, message): pass
+    def Error(self, message): pass
+    def Warning(self, message): pass
+    def Info(self, message): pass
+    def Debug(self, message): pass
+
+
+class KernelCallback(KernelCallbackLog):
+    """kernel/KernelInterface.h:279-304."""
+
+    def WriteFrame(self, frame, domain, data):
+        raise NotImplementedError
+
+    def WriteSample(self, startSample, receiver, data):
+        raise NotImplementedError
+
+
+class SimulationMetadata:
+    """kernel/KernelInterface.h:309-327."""
+
+    def __init__(self):
+        self.DomainMetadata = []
+        self.Framecount = 0
+        self.DomainPositions = []
+
+
+class KernelInterface:
+    """kernel/KernelInterface.h:332-351."""
+
+    def initialize_kernel(self, config, callbackLog): raise NotImplementedError
+    def run(self, callback): raise NotImplementedError
+    def get_metadata(self): raise NotImplementedError
+
+
+def _check(status, handle_error):
+    if status == T.PSTD_ERR_NOT_CONFIGURED:
+        raise PSTDKernelNotConfiguredException()
+    if status != T.PSTD_OK:
+        raise PSTDError(status, handle_error())
+
+
+def _callbacks(cb):
+    """Builds a pstd_callbacks struct (and keeps the ctypes thunks alive)."""
+    levels = ("Fatal", "Error", "Warning", "Info", "Debug")
+
+    def log(user, level, msg):
+        if cb is not None:
+            getattr(cb, levels[level] if 0 <= level < 5 else "Debug")(msg.decode())
+
+    def frame(user, f, d, data, n):
+        cb.WriteFrame(f, d, np.ctypeslib.as_array(data, shape=(n,)).copy())
+
+    def sample(user, s, r, data, n):
+        cb.WriteSample(s, r, [float(data[i]) for i in range(n)])
+
+    c = T.pstd_callbacks()
+    keep = [T.LOG_CB(log)]
+    c.log = keep[0]
+    if isinstance(cb, KernelCallback):
+        keep += [T.FRAME_CB(frame), T.SAMPLE_CB(sample)]
+        c.write_frame, c.write_sample = keep[1], keep[2]
+    return c, keep
+
+
+class PSTDKernel(KernelInterface):
+    """kernel/PSTDKernel.cpp:44-168 with the GPU solver slot filled."""
+
+    def __init__(self, GPU=True, MCPU=False):
+        self._h = C.c_void_p()
+        lib().pstd_kernel_create(int(bool(GPU)), int(bool(MCPU)), C.byref(self._h))
+        self._configured = False
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            lib().pstd_kernel_destroy(h)
+            self._h = None
+
+    def _err(self):
+        return lib().pstd_kernel_last_error(self._h).decode()
+
+    def initialize_kernel(self, config, callbackLog=None):
+        cconf, keep = config.to_c()
+        cbs, keep2 = _callbacks(callbackLog)
+        _check(lib().pstd_kernel_initialize(self._h, C.byref(cconf), C.byref(cbs)), self._err)
+        self._configured = True
+        self._config = config
+
+    def run(self, callback):
+        cbs, keep = _callbacks(callback)
+        _check(lib().pstd_kernel_run(self._h, C.byref(cbs)), self._err)
+
+    def get_metadata(self):
+        n = C.c_int32()
+        fc = C.c_int32()
+        _check(lib().pstd_kernel_get_metadata(self._h, C.byref(n), None, 0, C.byref(fc)), self._err)
+        sizes = (C.c_int32 * (3 * max(1, n.value)))()
+        _check(lib().pstd_kernel_get_metadata(self._h, C.byref(n), sizes, n.value, C.byref(fc)), self._err)
+        m = SimulationMetadata()
+        m.DomainMetadata = [[sizes[3 * i], sizes[3 * i + 1], sizes[3 * i + 2]] for i in range(n.value)]
+        m.Framecount = fc.value
+        return m
+
+    def prepare(self):
+        """Device context with the scene uploaded, frame 0 next (see DeviceSolver)."""
+        return DeviceSolver(self)
+
+    # ---- scene inspection (the test-only PSTDKernel::get_scene analogue)
+    def domain_count(self):
+        return len(self.get_metadata().DomainMetadata)
+
+    def domain_info(self, i):
+        info = (C.c_int32 * 16)()
+        f = (C.c_float * 3)()
+        _check(lib().pstd_kernel_domain_info(self._h, i, info, f), self._err)
+        keys = ["id", "tlx", "tly", "nx", "ny", "is_pml", "is_secondary_pml", "is_rigid", "local", "n_left",
+                "n_right", "n_top", "n_bottom", "update_x", "update_y", "pml_for_count"]
+        d = {k: int(info[j]) for j, k in enumerate(keys)}
+        d.update(alpha=f[0], impedance=f[1], rho=f[2])
+        return d
+
+    def _shape(self, i, name):
+        d = self.domain_info(i)
+        nx, ny = d["nx"], d["ny"]
+        return {"vx0": (ny, nx + 1), "vy0": (ny + 1, nx)}.get(name, (ny, nx))
+
+    def field(self, i, name):
+        shape = self._shape(i, name)
+        out = np.zeros(shape, dtype=np.float32)
+        _check(lib().pstd_kernel_read_field(self._h, i, T.FIELD_IDS[name], out.ctypes.data_as(C.POINTER(C.c_float)),
+                                            out.size), self._err)
+        return out
+
+    def pml(self, i, which):
+        shape = self._shape(i, ["px0", "py0", "vx0", "vy0"][which])
+        out = np.zeros(shape, dtype=np.float32)
+        _check(lib().pstd_kernel_read_pml(self._h, i, which, out.ctypes.data_as(C.POINTER(C.c_float)), out.size),
+               self._err)
+        return out
+
+    def segments(self):
+        n = lib().pstd_kernel_segment_count(self._h)
+        out = []
+        for i in range(max(0, n)):
+            s = T.pstd_plan_segment()
+            lib().pstd_kernel_get_segment(self._h, i, C.byref(s))
+            out.append({name: getattr(s, name) for name, _ in s._fields_})
+        return out
+
+
+class DeviceSolver:
+    """The device context of a configured PSTDKernel, driven step by step.
+
+    ``PSTDKernel.prepare()`` uploads the plan and the initial fields (frame 0
+    next); ``advance(n)`` runs n time steps and returns the receiver samples
+    (n x receivers), one per step as openPSTD's Receiver::received_values.
+    """
+
+    def __init__(self, kernel):
+        self.kernel = kernel  # owns the context
+        self.ctx = C.c_void_p()
+        _check(lib().pstd_kernel_prepare(kernel._h, C.byref(self.ctx)), kernel._err)
+        self.n_receivers = len(kernel._config.Receivers)
+
+    def _err(self):
+        return lib().pstd_ctx_last_error(self.ctx).decode()
+
+    def advance(self, steps):
+        out = np.zeros((steps, max(1, self.n_receivers)), dtype=np.float32)
+        ptr = out.ctypes.data_as(C.POINTER(C.c_float)) if self.n_receivers else None
+        _check(lib().pstd_ctx_advance(self.ctx, steps, ptr), self._err)
+        return out[:, : self.n_receivers]
+
+    def advance_async(self, steps):
+        _check(lib().pstd_ctx_advance_async(self.ctx, steps), self._err)
+
+    def synchronize(self):
+        _check(lib().pstd_ctx_synchronize(self.ctx), self._err)
+
+    @property
+    def steps_done(self):
+        return lib().pstd_ctx_steps_done(self.ctx)
+
+    def field(self, i, name):
+        out = np.zeros(self.kernel._shape(i, name), dtype=np.float32)
+        _check(lib().pstd_ctx_get_field(self.ctx, i, T.FIELD_IDS[name], out.ctypes.data_as(C.POINTER(C.c_float)),
+                                        out.size), self._err)
+        return out
+
+    def set_field(self, i, name, values):
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        _check(lib().pstd_ctx_set_field(self.ctx, i, T.FIELD_IDS[name], v.ctypes.data_as(C.POINTER(C.c_float)),
+                                        v.size), self._err)
+
+
+class RecordingCallback(KernelCallback):
+    """Collects frames and samples in memory (the CLI's PSTDFile output in
+    CLI/output.cpp:30-67 writes the same calls to disk)."""
+
+    def __init__(self):
+        self.frames = []   # (frame, domain, ndarray)
+        self.samples = {}  # receiver -> list of (startSample, value)
+        self.messages = []
+
+    def WriteFrame(self, frame, domain, data):
+        self.frames.append((frame, domain, data))
+
+    def WriteSample(self, startSample, receiver, data):
+        self.samples.setdefault(receiver, []).append((startSample, data[0]))
+
+    def Info(self, message):
+        self.messages.append(("Info", message))
+
+    def Error(self, message):
+        self.messages.append(("Error", message))
+
+    def Debug(self, message):
+        self.messages.append(("Debug", message))

@@ -1,0 +1,1415 @@
+// TEST INFRASTRUCTURE ONLY — CPU oracle of openpstd_amd (see pstd_oracle.h).
+//
+// A restatement, written from the method, of openPSTD's PSTD update loop:
+//   * next_2_power, get_rho_array, get_window_coefficients
+//       (kernel/core/kernel_functions.cpp:37-72, 327-333)
+//   * WisdomCache wavenumber discretisation (kernel/core/WisdomCache.cpp:39-75, 113-115)
+//   * spatderp3 windowed FFT derivative (kernel/core/kernel_functions.cpp:123-318)
+//   * Domain: construction, calc segment loop, PML arrays, update directions
+//       (kernel/core/Domain.cpp:53-92, 95-391, 543-616, 644-780)
+//   * Scene: neighbour graph, primary/secondary PML construction, speakers,
+//       receivers (kernel/core/Scene.cpp:41-380)
+//   * Speaker Gaussian initial state (kernel/core/Speaker.cpp:21-39)
+//   * Receiver nearest-cell sampling (kernel/core/Receiver.cpp:34-83)
+//   * Solver RK6 loop (kernel/Solver.cpp:62-182)
+//   * PSTDKernel scene set-up (kernel/PSTDKernel.cpp:49-119, 170-190)
+//
+// Arithmetic is templated: R = float follows the reference's single-precision
+// operation order; R = double is the same method in FP64.  Scene geometry
+// (grid scaling, rigid test, receiver cells, speaker grid position) is
+// computed in float for both, so both instances simulate identical grids.
+// The FFT is the oracle's own radix-2 transform (FFTW is not available here).
+#include "pstd_oracle.h"
+
+#include <algorithm>
+#include <cassert>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <iterator>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+static thread_local std::string g_error;
+
+enum Direction { LEFT = 0, RIGHT = 1, TOP = 2, BOTTOM = 3 };
+// kernel/core/kernel_functions.h:92-93
+static const Direction kAllDirections[4] = {LEFT, RIGHT, TOP, BOTTOM};
+enum CalcDir { X = 0, Y = 1 };
+enum CalcType { PRESSURE = 0, VELOCITY = 1 };
+static const float kEpsilon = 1e-5f;  // kernel/core/kernel_functions.h:65
+
+struct Point {
+    int x = 0, y = 0, z = 0;
+    Point() = default;
+    Point(int x_, int y_, int z_ = 0) : x(x_), y(y_), z(z_) {}
+    Point operator+(const Point &o) const { return Point(x + o.x, y + o.y, z + o.z); }
+    Point operator-(const Point &o) const { return Point(x - o.x, y - o.y, z - o.z); }
+    int at(int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+// Row-major dense matrix; (r, c) = (y, x) for domain fields.
+template <class R>
+struct Mat {
+    int rows = 0, cols = 0;
+    std::vector<R> a;
+    Mat() = default;
+    Mat(int r, int c) : rows(r), cols(c), a((size_t)r * c, R(0)) {}
+    R &operator()(int r, int c) { return a[(size_t)r * cols + c]; }
+    const R &operator()(int r, int c) const { return a[(size_t)r * cols + c]; }
+    Mat block(int r0, int c0, int nr, int nc) const {
+        if (r0 < 0 || c0 < 0 || r0 + nr > rows || c0 + nc > cols)
+            throw std::runtime_error("matrix block out of range");
+        Mat m(nr, nc);
+        for (int r = 0; r < nr; ++r)
+            for (int c = 0; c < nc; ++c) m(r, c) = (*this)(r0 + r, c0 + c);
+        return m;
+    }
+    void set_block(int r0, int c0, const Mat &m) {
+        if (r0 < 0 || c0 < 0 || r0 + m.rows > rows || c0 + m.cols > cols)
+            throw std::runtime_error("matrix set_block out of range");
+        for (int r = 0; r < m.rows; ++r)
+            for (int c = 0; c < m.cols; ++c) (*this)(r0 + r, c0 + c) = m(r, c);
+    }
+    Mat transposed() const {
+        Mat t(cols, rows);
+        for (int r = 0; r < rows; ++r)
+            for (int c = 0; c < cols; ++c) t(c, r) = (*this)(r, c);
+        return t;
+    }
+};
+
+// ---------------------------------------------------------------- primitives
+// kernel/core/kernel_functions.cpp:66-72: max(2^ceil(log2 n), 1)
+static int next_2_power_int(int n) { return std::max((int)std::pow(2.0, std::ceil(std::log2((double)n))), 1); }
+static int next_2_power_float(float n) { return std::max((int)std::pow(2.0, std::ceil(std::log2(n))), 1); }
+
+template <class R>
+struct RhoArray {
+    R p[4][2];
+    R v[4][2];
+};
+
+// kernel/core/kernel_functions.cpp:37-64
+template <class R>
+static RhoArray<R> get_rho_array(R rho1, R rho_self, R rho2) {
+    R zn1 = rho1 / rho_self, inv_zn1 = rho_self / rho1;
+    R rlw1 = (zn1 - 1) / (zn1 + 1), rlw2 = (inv_zn1 - 1) / (inv_zn1 + 1);
+    R tlw1 = (2 * zn1) / (zn1 + 1), tlw2 = (2 * inv_zn1) / (inv_zn1 + 1);
+    R zn2 = rho2 / rho_self, inv_zn2 = rho_self / rho2;
+    R rrw1 = (zn2 - 1) / (zn2 + 1), rrw2 = (inv_zn2 - 1) / (inv_zn2 + 1);
+    R trw1 = (2 * zn2) / (zn2 + 1), trw2 = (2 * inv_zn2) / (inv_zn2 + 1);
+    RhoArray<R> r;
+    R p[4][2] = {{rlw1, rlw2}, {rrw1, rrw2}, {tlw1, tlw2}, {trw1, trw2}};
+    R v[4][2] = {{-rlw1, -rlw2}, {-rrw1, -rrw2}, {tlw2, tlw1}, {trw2, trw1}};
+    std::memcpy(r.p, p, sizeof p);
+    std::memcpy(r.v, v, sizeof v);
+    return r;
+}
+
+// kernel/core/kernel_functions.cpp:327-333:
+// exp(-ln(10) * alpha * x^6), x = linspace(-w, w, 2w+1)/w, alpha = (pe-40)/20+1
+template <class R>
+static std::vector<R> get_window_coefficients(int w, int patch_error) {
+    R alpha = (R)((patch_error - 40) / 20.0 + 1);
+    R ln10 = (R)std::log(10.0);
+    std::vector<R> out(2 * w + 1);
+    R step = (R)(w - (-w)) / (R)(2 * w);  // LinSpaced step
+    for (int i = 0; i <= 2 * w; ++i) {
+        R x = ((R)(-w) + (R)i * step) / (R)w;
+        R x2 = x * x;
+        R x6 = x2 * x2 * x2;
+        out[i] = std::exp(x6 * ln10 * alpha * (R)(-1));
+    }
+    return out;
+}
+
+template <class R>
+struct Discretization {
+    std::vector<R> wave_numbers;
+    std::vector<std::complex<R>> complex_factors, pressure, velocity;
+};
+
+// kernel/core/WisdomCache.cpp:53-75 with N the exponent (match_number, :113-115).
+template <class R>
+static Discretization<R> discretize_wave_numbers(R dx, int exponent) {
+    R kmax = (R)M_PI / dx;
+    int tp = (int)std::pow(2.0, exponent - 1);
+    R dka = kmax / (R)tp;
+    Discretization<R> d;
+    int L = 2 * tp;
+    d.wave_numbers.assign(L, 0);
+    // head(tp+1) = LinSpaced(tp+1, 0, kmax)
+    R step1 = tp >= 1 ? (kmax - 0) / (R)tp : 0;
+    for (int i = 0; i <= tp; ++i) d.wave_numbers[i] = (R)0 + (R)i * step1;
+    // tail(tp-1) = LinSpaced(tp-1, kmax-dka, dka)
+    int nt = tp - 1;
+    if (nt > 0) {
+        R lo = kmax - dka;
+        R step2 = nt > 1 ? (dka - lo) / (R)(nt - 1) : 0;
+        for (int i = 0; i < nt; ++i) d.wave_numbers[tp + 1 + i] = lo + (R)i * step2;
+    }
+    d.complex_factors.assign(L, std::complex<R>(0, 1));
+    for (int i = tp + 1; i < L; ++i) d.complex_factors[i] = std::complex<R>(0, -1);
+    d.pressure.resize(L);
+    d.velocity.resize(L);
+    R half = dx * (R)0.5;
+    for (int i = 0; i < L; ++i) {
+        std::complex<R> cwn = d.complex_factors[i] * d.wave_numbers[i];
+        d.pressure[i] = std::exp(-cwn * std::complex<R>(half, 0)) * cwn;
+        d.velocity[i] = std::exp(cwn * std::complex<R>(half, 0)) * cwn;
+    }
+    return d;
+}
+
+static int match_number(int n) { return (int)std::ceil(std::log2((double)n)); }
+
+// ---------------------------------------------------------------- FFT
+// Iterative radix-2 complex FFT; twiddles from double-precision sin/cos.
+template <class R>
+struct ComplexFFT {
+    int n = 0, logn = 0;
+    std::vector<std::complex<R>> tw;  // e^{-2 pi i k / n}, k < n/2
+    std::vector<int> rev;
+    explicit ComplexFFT(int n_) : n(n_) {
+        while ((1 << logn) < n) ++logn;
+        if ((1 << logn) != n) throw std::runtime_error("FFT length must be a power of two");
+        tw.resize(std::max(1, n / 2));
+        for (int k = 0; k < n / 2; ++k) {
+            double a = -2.0 * M_PI * (double)k / (double)n;
+            tw[k] = std::complex<R>((R)std::cos(a), (R)std::sin(a));
+        }
+        rev.resize(n);
+        for (int i = 0; i < n; ++i) {
+            int r = 0;
+            for (int b = 0; b < logn; ++b)
+                if (i & (1 << b)) r |= 1 << (logn - 1 - b);
+            rev[i] = r;
+        }
+    }
+    void run(std::complex<R> *a, bool inverse) const {
+        for (int i = 0; i < n; ++i)
+            if (i < rev[i]) std::swap(a[i], a[rev[i]]);
+        for (int len = 2; len <= n; len <<= 1) {
+            int half = len >> 1, stride = n / len;
+            for (int s = 0; s < n; s += len) {
+                for (int j = 0; j < half; ++j) {
+                    std::complex<R> w = tw[j * stride];
+                    if (inverse) w = std::conj(w);
+                    std::complex<R> u = a[s + j];
+                    std::complex<R> v = a[s + j + half] * w;
+                    a[s + j] = u + v;
+                    a[s + j + half] = u - v;
+                }
+            }
+        }
+    }
+};
+
+// Real transforms of length F through a complex transform of length F/2.
+template <class R>
+struct RealFFT {
+    int F;
+    ComplexFFT<R> half;
+    std::vector<std::complex<R>> post;  // e^{-2 pi i k / F}, k < F/2
+    explicit RealFFT(int F_) : F(F_), half(F_ / 2) {
+        post.resize(F / 2);
+        for (int k = 0; k < F / 2; ++k) {
+            double a = -2.0 * M_PI * (double)k / (double)F;
+            post[k] = std::complex<R>((R)std::cos(a), (R)std::sin(a));
+        }
+    }
+    // r2c: X[0..F/2] of the unnormalised DFT (FFTW r2c convention).
+    void forward(const R *x, std::complex<R> *X, std::vector<std::complex<R>> &z) const {
+        int M = F / 2;
+        z.resize(M);
+        for (int m = 0; m < M; ++m) z[m] = std::complex<R>(x[2 * m], x[2 * m + 1]);
+        half.run(z.data(), false);
+        for (int k = 0; k <= M; ++k) {
+            std::complex<R> zk = z[k % M];
+            std::complex<R> zc = std::conj(z[(M - k) % M]);
+            std::complex<R> e = (zk + zc) * (R)0.5;
+            std::complex<R> o = (zk - zc) * std::complex<R>(0, (R)-0.5);
+            std::complex<R> w = k < M ? post[k] : std::complex<R>(-1, 0);
+            X[k] = e + w * o;
+        }
+    }
+    // c2r (FFTW convention: unnormalised, imaginary parts of bins 0 and F/2 ignored).
+    void inverse(const std::complex<R> *Xin, R *y, std::vector<std::complex<R>> &z) const {
+        int M = F / 2;
+        z.resize(M);
+        for (int k = 0; k < M; ++k) {
+            std::complex<R> a = Xin[k], b = Xin[M - k];
+            if (k == 0) {
+                a = std::complex<R>(a.real(), 0);
+                b = std::complex<R>(b.real(), 0);
+            }
+            std::complex<R> e = a + std::conj(b);
+            std::complex<R> o = (a - std::conj(b)) * std::conj(post[k]);
+            z[k] = e + std::complex<R>(0, 1) * o;
+        }
+        half.run(z.data(), true);
+        for (int m = 0; m < M; ++m) {
+            y[2 * m] = z[m].real();
+            y[2 * m + 1] = z[m].imag();
+        }
+    }
+};
+
+template <class R>
+struct FFTCache {
+    std::mutex mu;
+    std::map<int, std::shared_ptr<RealFFT<R>>> plans;
+    std::shared_ptr<RealFFT<R>> get(int F) {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = plans.find(F);
+        if (it != plans.end()) return it->second;
+        auto p = std::make_shared<RealFFT<R>>(F);
+        plans[F] = p;
+        return p;
+    }
+};
+
+// ---------------------------------------------------------------- spatderp3
+// kernel/core/kernel_functions.cpp:123-318.  p1, p2, p3 are the
+// side/main/side blocks; for Y they are transposed first so lines are rows.
+template <class R>
+static Mat<R> spatderp3(Mat<R> p1, Mat<R> p2, Mat<R> p3, const std::vector<std::complex<R>> &derfact,
+                        const RhoArray<R> &rho, const std::vector<R> &window, int wlen, CalcType ct,
+                        CalcDir direct, FFTCache<R> &fftc) {
+    if (direct == Y) {
+        p1 = p1.transposed();
+        p2 = p2.transposed();
+        p3 = p3.transposed();
+    }
+    const int batch = p2.rows;
+    const int n2 = p2.cols;
+    const int F = next_2_power_int(n2 + wlen * 2);
+    if ((int)derfact.size() < F / 2 + 1)
+        throw std::runtime_error("derivative factor table shorter than F/2+1 (wavenumber/FFT length mismatch)");
+    auto fft = fftc.get(F);
+    const int outn = ct == PRESSURE ? n2 + 1 : n2 - 1;
+    Mat<R> result(batch, outn);
+    std::vector<R> line(F), y(F);
+    std::vector<std::complex<R>> spec(F / 2 + 1), scratch;
+    const R *wl = window.data();
+    const R *wr = window.data() + wlen + 1;
+    for (int r = 0; r < batch; ++r) {
+        std::fill(line.begin(), line.end(), R(0));
+        if (ct == PRESSURE) {
+            if (p1.cols < wlen || p3.cols < wlen) throw std::runtime_error("window bigger than sides");
+            for (int j = 0; j < wlen; ++j) {
+                R d1 = p1(r, p1.cols - wlen + j) * rho.p[2][1] + p2(r, wlen - 1 - j) * rho.p[0][0];
+                line[j] = d1 * wl[j];
+                R d3 = p3(r, j) * rho.p[3][1] + p2(r, n2 - 1 - j) * rho.p[1][0];
+                line[wlen + n2 + j] = d3 * wr[j];
+            }
+        } else {
+            if (p1.cols < wlen + 1 || p3.cols < wlen + 1) throw std::runtime_error("window bigger than sides");
+            for (int j = 0; j < wlen; ++j) {
+                R d1 = p1(r, p1.cols - wlen - 1 + j) * rho.v[2][1] + p2(r, wlen - j) * rho.v[0][0];
+                line[j] = d1 * wl[j];
+                R d3 = p3(r, 1 + j) * rho.v[3][1] + p2(r, n2 - 2 - j) * rho.v[1][0];
+                line[wlen + n2 + j] = d3 * wr[j];
+            }
+        }
+        for (int i = 0; i < n2; ++i) line[wlen + i] = p2(r, i);
+        fft->forward(line.data(), spec.data(), scratch);
+        for (int k = 0; k <= F / 2; ++k) spec[k] = spec[k] * derfact[k];
+        fft->inverse(spec.data(), y.data(), scratch);
+        for (int i = 0; i < outn; ++i) result(r, i) = y[wlen + i] / (R)F;
+    }
+    if (direct == Y) result = result.transposed();
+    return result;
+}
+
+// ---------------------------------------------------------------- settings
+struct Settings {
+    pstd_settings s;
+    float dt() const { return s.tfactRK * s.grid_spacing / s.c1; }  // KernelInterface.cpp:125-127
+    int window_size() const {                                      // KernelInterface.cpp:129-134
+        int tmp = (int)std::round((s.patcherror * 0.7 - 17) / 2.);
+        tmp = (int)std::round(tmp * 2);
+        return tmp;
+    }
+};
+
+struct EdgeParameters {
+    bool locally_reacting = false;
+    float alpha = 0;
+};
+
+// ---------------------------------------------------------------- domain
+template <class R>
+struct FieldValues {
+    Mat<R> vx0, vy0, p0, px0, py0;
+};
+template <class R>
+struct FieldLValues {
+    Mat<R> Lpx, Lpy, Lvx, Lvy;
+};
+template <class R>
+struct PMLArrays {
+    Mat<R> px, py, vx, vy;
+};
+
+template <class R>
+struct Scene;
+
+template <class R>
+struct Domain {
+    Scene<R> *scene = nullptr;
+    int id = 0;
+    float alpha = 0;
+    float impedance = 0;  // geometry decisions (rigid) use the reference's float value
+    R impedance_r = 0;
+    R rho = 0;
+    std::map<Direction, EdgeParameters> edge_param_map;
+    bool should_update[2] = {true, true};
+    Point top_left, bottom_right, size;
+    bool is_pml = false, local = false, is_secondary_pml = false;
+    FieldValues<R> current, previous;
+    FieldLValues<R> L;
+    std::vector<Domain *> pml_for;
+    std::vector<Domain *> left, right, top, bottom;
+    int num_neighbour_domains = 0, num_pml_neighbour_domains = 0;
+    bool has_horizontal_attenuation = false, is_corner_domain = false;
+    std::vector<bool> needs_reversed_attenuation;
+    PMLArrays<R> pml;
+
+    Mat<R> extended_zeros(int y, int x) const { return Mat<R>(size.y + y, size.x + x); }
+
+    bool is_rigid() const { return impedance > 1000; }  // Domain.cpp:431-433
+
+    std::vector<int> get_range(CalcDir cd) const {  // Domain.cpp:435-448
+        int a = cd == X ? top_left.y : top_left.x;
+        int b = cd == X ? bottom_right.y : bottom_right.x;
+        std::vector<int> t(std::max(0, b - a));
+        std::iota(t.begin(), t.end(), a);
+        return t;
+    }
+
+    std::vector<Domain *> &neighbours_at(Direction d) {
+        switch (d) {
+            case LEFT: return left;
+            case RIGHT: return right;
+            case TOP: return top;
+            default: return bottom;
+        }
+    }
+
+    bool is_neighbour_of(Domain *o) {
+        for (Direction d : kAllDirections) {
+            auto &v = neighbours_at(d);
+            if (std::find(v.begin(), v.end(), o) != v.end()) return true;
+        }
+        return false;
+    }
+    bool is_pml_for(Domain *o) const { return std::find(pml_for.begin(), pml_for.end(), o) != pml_for.end(); }
+
+    bool contains_location(const float loc[3]) const {  // Domain.cpp:407-415
+        for (int dim = 0; dim < 3; ++dim)
+            if ((float)top_left.at(dim) > loc[dim] || loc[dim] > (float)bottom_right.at(dim)) return false;
+        return true;
+    }
+
+    std::vector<int> intersection_with(Domain *o, Direction d) {  // Domain.cpp:450-470
+        CalcDir cd = (d == LEFT || d == RIGHT) ? X : Y;
+        std::vector<int> a = get_range(cd), b = o->get_range(cd), out;
+        std::set_intersection(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(out));
+        return out;
+    }
+
+    void add_neighbour_at(Domain *d, Direction dir) { neighbours_at(dir).push_back(d); }
+
+    int num_pmls_in_direction(Direction d) {
+        int n = 0;
+        for (auto *o : neighbours_at(d))
+            if (o->is_pml) ++n;
+        return n;
+    }
+
+    void compute_number_of_neighbours() {  // Domain.cpp:504-515
+        num_neighbour_domains = num_pml_neighbour_domains = 0;
+        for (Direction d : kAllDirections)
+            for (auto *o : neighbours_at(d)) {
+                ++num_neighbour_domains;
+                if (o->is_pml) ++num_pml_neighbour_domains;
+            }
+    }
+    int number_of_neighbours(bool count_pml) const {
+        return count_pml ? num_neighbour_domains : num_neighbour_domains - num_pml_neighbour_domains;
+    }
+
+    // Domain.cpp:543-584 (single vacant segment per edge; openPSTD corrupts
+    // rows for several segments, so that case is rejected).
+    std::vector<std::pair<int, int>> get_vacant_range(Direction d) {
+        CalcDir cd = (d == LEFT || d == RIGHT) ? X : Y;
+        std::vector<int> range = get_range(cd);
+        std::set<int> rs(range.begin(), range.end());
+        for (auto *o : neighbours_at(d)) {
+            std::vector<int> nr = o->get_range(cd);
+            std::set<int> ns(nr.begin(), nr.end()), diff;
+            std::set_difference(rs.begin(), rs.end(), ns.begin(), ns.end(), std::inserter(diff, diff.end()));
+            rs.swap(diff);
+        }
+        std::vector<int> v(rs.begin(), rs.end());
+        std::vector<std::pair<int, int>> out;
+        for (size_t i = 0; i < v.size(); ++i) {
+            if (i == 0 || v[i - 1] + 1 < v[i]) out.push_back({v[i], v[i]});
+            if (i == v.size() - 1 || v[i + 1] - 1 > v[i]) out.back().second = v[i] + 1;
+        }
+        if (out.size() > 1)
+            throw std::runtime_error("edge with more than one vacant segment (openPSTD get_vacant_range corrupts it)");
+        return out;
+    }
+
+    void find_update_directions() {  // Domain.cpp:587-616
+        for (CalcDir cd : {X, Y}) {
+            bool su = true;
+            if (number_of_neighbours(false) == 1 && is_pml) {
+                if (has_horizontal_attenuation && cd == X) {
+                    su = true;
+                } else if (!has_horizontal_attenuation && cd == Y) {
+                    su = true;
+                } else if (local) {
+                    su = false;
+                } else {
+                    for (Direction d : kAllDirections) {
+                        auto &nb = neighbours_at(d);
+                        if (nb.size() == 1 && !nb[0]->is_pml) {
+                            Direction opp = d == LEFT ? RIGHT : d == RIGHT ? LEFT : d == TOP ? BOTTOM : TOP;
+                            auto &back = nb[0]->neighbours_at(opp);
+                            if (!(back.size() == 1 && back[0]->id == id))
+                                throw std::runtime_error("inconsistent neighbour graph");
+                            su = !nb[0]->edge_param_map[d].locally_reacting;
+                        }
+                    }
+                }
+            }
+            should_update[cd] = su;
+        }
+    }
+
+    // The attenuation flags compute_pml_matrices sets (Domain.cpp:650-686).
+    void compute_attenuation_flags() {
+        needs_reversed_attenuation.clear();
+        if (is_secondary_pml) {
+            is_corner_domain = num_neighbour_domains == 2;
+            has_horizontal_attenuation = !left.empty() || !right.empty();
+            if (is_corner_domain) {
+                if (!(has_horizontal_attenuation && (!top.empty() || !bottom.empty())))
+                    throw std::runtime_error("corner PML without horizontal and vertical neighbours");
+                needs_reversed_attenuation.push_back(!left.empty());
+                needs_reversed_attenuation.push_back(!bottom.empty());
+            } else {
+                needs_reversed_attenuation.push_back(!left.empty() || !bottom.empty());
+            }
+        } else {
+            is_corner_domain = false;
+            bool all_air_left = !left.empty() && num_pmls_in_direction(LEFT) == 0;
+            bool all_air_right = !right.empty() && num_pmls_in_direction(RIGHT) == 0;
+            bool all_air_bottom = !bottom.empty() && num_pmls_in_direction(BOTTOM) == 0;
+            has_horizontal_attenuation = all_air_left || all_air_right;
+            needs_reversed_attenuation.push_back(all_air_left || all_air_bottom);
+        }
+    }
+
+    void create_attenuation_array(CalcDir cd, bool ascending, Mat<R> &pp, Mat<R> &pv);
+    void compute_pml_matrices();
+    void calc(CalcDir cd, CalcType ct);
+};
+
+template <class R>
+struct Speaker {
+    float x, y, z;
+};
+
+template <class R>
+struct Receiver {
+    float loc[3];
+    Point grid;
+    Domain<R> *container = nullptr;
+    int id = 0;
+    std::vector<R> received;
+};
+
+template <class R>
+struct Scene {
+    Settings settings;
+    std::vector<std::unique_ptr<Domain<R>>> storage;
+    std::vector<Domain<R> *> domain_list;
+    std::vector<Receiver<R>> receivers;
+    std::vector<Speaker<R>> speakers;
+    int number_of_domains = 0;
+    int receiver_mode = PSTD_RECEIVER_PHYSICAL;
+    int threads = 1;
+    int frame = 0;
+    std::mutex disc_mu;
+    std::map<int, Discretization<R>> disc_cache;
+    FFTCache<R> fftc;
+
+    int get_new_id() { return number_of_domains++; }
+
+    const Discretization<R> &get_discretization(int N) {  // WisdomCache.cpp:39-50
+        std::lock_guard<std::mutex> g(disc_mu);
+        int m = match_number(N);
+        auto it = disc_cache.find(m);
+        if (it != disc_cache.end()) return it->second;
+        disc_cache[m] = discretize_wave_numbers<R>((R)settings.s.grid_spacing, m);
+        return disc_cache[m];
+    }
+
+    Domain<R> *make_domain(int id, float alpha, Point tl, Point size, bool is_pml,
+                           std::map<Direction, EdgeParameters> epm, Domain<R> *pml_for) {
+        // Domain::initialize_domain (Domain.cpp:53-92)
+        auto d = std::make_unique<Domain<R>>();
+        d->scene = this;
+        d->top_left = tl;
+        d->size = size;
+        d->bottom_right = tl + size;
+        d->id = id;
+        d->edge_param_map = epm;
+        d->alpha = alpha;
+        d->impedance = -((std::sqrt(1 - alpha) + 1) / (std::sqrt(1 - alpha) - 1));
+        d->impedance_r = -((std::sqrt((R)1 - (R)alpha) + 1) / (std::sqrt((R)1 - (R)alpha) - 1));
+        if (is_pml) d->pml_for.push_back(pml_for);
+        d->rho = d->is_rigid() ? (R)1e30 : (R)settings.s.rho * d->impedance_r;
+        d->is_pml = is_pml;
+        d->is_secondary_pml = false;
+        for (auto *p : d->pml_for)
+            if (p->is_pml) d->is_secondary_pml = true;
+        d->current.p0 = d->extended_zeros(0, 0);
+        d->current.px0 = d->extended_zeros(0, 0);
+        d->current.py0 = d->extended_zeros(0, 0);
+        d->current.vx0 = d->extended_zeros(0, 1);
+        d->current.vy0 = d->extended_zeros(1, 0);
+        d->L.Lpx = d->extended_zeros(0, 1);
+        d->L.Lpy = d->extended_zeros(1, 0);
+        d->L.Lvx = d->extended_zeros(0, 0);
+        d->L.Lvy = d->extended_zeros(0, 0);
+        d->pml.px = d->extended_zeros(0, 0);
+        d->pml.py = d->extended_zeros(0, 0);
+        d->pml.vx = d->extended_zeros(0, 1);
+        d->pml.vy = d->extended_zeros(1, 0);
+        d->local = false;
+        Domain<R> *raw = d.get();
+        storage.push_back(std::move(d));
+        return raw;
+    }
+
+    // Scene::add_domain (Scene.cpp:297-380)
+    void add_domain(Domain<R> *domain) {
+        for (Domain<R> *other : domain_list) {
+            if (domain->is_secondary_pml && other->is_secondary_pml) continue;
+            if ((domain->is_secondary_pml && !other->is_pml) || (other->is_secondary_pml && !domain->is_pml)) continue;
+            if (domain->is_pml && other->is_pml) {
+                bool pml_for_domain = other->is_pml_for(domain);
+                bool pml_for_other = domain->is_pml_for(other);
+                if ((domain->is_secondary_pml && pml_for_other) || (other->is_secondary_pml && pml_for_domain)) {
+                    // domain is PML for the other: pass
+                } else if (other->pml_for.size() != 1 || domain->pml_for.size() != 1) {
+                    continue;
+                } else if (!other->pml_for[0]->is_neighbour_of(domain->pml_for[0])) {
+                    continue;
+                } else if (!domain->pml_for[0]->is_neighbour_of(other->pml_for[0])) {
+                    continue;
+                }
+            }
+            Direction orientation = LEFT;
+            bool is_neighbour = true;
+            if (domain->bottom_right.x == other->top_left.x) orientation = RIGHT;
+            else if (domain->top_left.x == other->bottom_right.x) orientation = LEFT;
+            else if (domain->bottom_right.y == other->top_left.y) orientation = BOTTOM;
+            else if (domain->top_left.y == other->bottom_right.y) orientation = TOP;
+            else is_neighbour = false;
+            bool other_pml_for_different = other->is_pml && !domain->is_pml && !other->is_pml_for(domain);
+            bool domain_pml_for_different = domain->is_pml && !other->is_pml && !domain->is_pml_for(other);
+            if (is_neighbour && !other_pml_for_different && !domain_pml_for_different) {
+                if (!domain->intersection_with(other, orientation).empty()) {
+                    Direction opp = orientation == LEFT ? RIGHT : orientation == RIGHT ? LEFT
+                                    : orientation == TOP ? BOTTOM : TOP;
+                    domain->add_neighbour_at(other, orientation);
+                    other->add_neighbour_at(domain, opp);
+                }
+            }
+        }
+        domain_list.push_back(domain);
+    }
+
+    static Domain<R> *singular_parent(Domain<R> *d) {
+        if (d && d->is_pml && d->pml_for.size() == 1) return d->pml_for[0];
+        return nullptr;
+    }
+
+    // Scene::add_pml_domains (Scene.cpp:41-225)
+    void add_pml_domains() {
+        const int N = settings.s.pml_cells;
+        const Direction dirs[4] = {LEFT, TOP, RIGHT, BOTTOM};
+        std::vector<Domain<R> *> first_order;
+        // openPSTD keys this map by shared_ptr address; allocation order is used here.
+        std::vector<std::pair<Domain<R> *, Direction>> second_order;
+        std::vector<Domain<R> *> snapshot = domain_list;
+        for (Domain<R> *domain : snapshot) {
+            if (domain->is_pml) continue;
+            for (int i = 0; i < 4; ++i) {
+                Direction direction = dirs[i];
+                auto vac = domain->get_vacant_range(direction);
+                for (auto &vr : vac) {
+                    int pml_id = get_new_id();
+                    float alpha = domain->edge_param_map[direction].alpha;
+                    float pml_alpha = std::max(alpha, kEpsilon);
+                    int xo = 0, yo = 0, xs = 0, ys = 0;
+                    switch (direction) {
+                        case LEFT: xo = -N; yo = vr.first - domain->top_left.y; xs = N; ys = vr.second - vr.first; break;
+                        case RIGHT: xo = domain->size.x; yo = vr.first - domain->top_left.y; xs = N; ys = vr.second - vr.first; break;
+                        case TOP: xo = vr.first - domain->top_left.x; yo = -N; xs = vr.second - vr.first; ys = N; break;
+                        case BOTTOM: xo = vr.first - domain->top_left.x; yo = domain->size.y; xs = vr.second - vr.first; ys = N; break;
+                    }
+                    Point off(xo, yo);
+                    Point ptl = off + domain->top_left;
+                    CalcDir cdir = (direction == LEFT || direction == RIGHT) ? X : Y;
+                    bool full = cdir == X ? (vr.first == domain->top_left.y && vr.second == domain->bottom_right.y)
+                                          : (vr.first == domain->top_left.x && vr.second == domain->bottom_right.x);
+                    Domain<R> *pml = make_domain(pml_id, pml_alpha, ptl, Point(xs, ys), true, {}, domain);
+                    first_order.push_back(pml);
+                    if (!full) pml->local = true;
+                    if (alpha > 0 && full) {
+                        for (int sd : {(i + 1) % 4, (i + 3) % 4}) {
+                            Direction second_dir = dirs[sd];
+                            int sid = get_new_id();
+                            float other_alpha = domain->edge_param_map[second_dir].alpha;
+                            float sa = std::min(std::max(kEpsilon, other_alpha), std::max(kEpsilon, pml_alpha));
+                            int sx = 0, sy = 0;
+                            switch (second_dir) {
+                                case LEFT: sx = -N; sy = 0; break;
+                                case RIGHT: sx = pml->size.x; sy = 0; break;
+                                case TOP: sx = 0; sy = -N; break;
+                                case BOTTOM: sx = 0; sy = pml->size.y; break;
+                            }
+                            Point stl = domain->top_left + off + Point(sx, sy);
+                            Domain<R> *sec = make_domain(sid, sa, stl, Point(N, N), true, {}, pml);
+                            second_order.push_back({sec, second_dir});
+                        }
+                    }
+                }
+            }
+        }
+        for (auto *d : first_order) add_domain(d);
+        std::map<std::vector<int>, std::vector<Domain<R> *>> by_corner;
+        for (auto &e : second_order) {
+            Domain<R> *parent = e.first->pml_for.at(0);
+            if (!parent->neighbours_at(e.second).empty()) continue;
+            std::vector<int> cp = {e.first->top_left.x, e.first->top_left.y, e.first->bottom_right.x,
+                                   e.first->bottom_right.y};
+            by_corner[cp].push_back(e.first);
+        }
+        std::vector<Domain<R> *> sec_list;
+        for (auto &e : by_corner) {
+            auto &v = e.second;
+            if (v.size() == 1) {
+                sec_list.push_back(v[0]);
+                continue;
+            }
+            std::vector<Domain<R> *> processed;
+            for (size_t i = 0; i < v.size(); ++i)
+                for (size_t j = i + 1; j < v.size(); ++j) {
+                    Domain<R> *di = v[i], *dj = v[j];
+                    bool pi = std::find(processed.begin(), processed.end(), di) != processed.end();
+                    bool pj = std::find(processed.begin(), processed.end(), dj) != processed.end();
+                    if (pi || pj) continue;
+                    Domain<R> *p1 = singular_parent(singular_parent(di));
+                    Domain<R> *p2 = singular_parent(singular_parent(dj));
+                    if (p1 != nullptr && p2 != nullptr && p1->id == p2->id) {
+                        processed.push_back(di);
+                        processed.push_back(dj);
+                        for (auto *pf : dj->pml_for) di->pml_for.push_back(pf);
+                        sec_list.push_back(di);
+                    }
+                }
+            for (auto *d : v)
+                if (std::find(processed.begin(), processed.end(), d) == processed.end()) sec_list.push_back(d);
+        }
+        for (auto *d : sec_list) add_domain(d);
+    }
+
+    // Speaker::addDomainContribution (Speaker.cpp:21-39); SQR(x) expands to x*x without parentheses.
+    void add_speaker(float x, float y, float z) {
+        float dx2 = 0.5f;
+        Speaker<R> sp{x - dx2, y - dx2, z - dx2};
+        for (Domain<R> *d : domain_list) {
+            R dx = (R)settings.s.grid_spacing;
+            R rel_x = (R)(sp.x - (float)d->top_left.x);
+            R rel_y = (R)(sp.y - (float)d->top_left.y);
+            if (sizeof(R) == sizeof(float)) {
+                rel_x = (R)(float)(sp.x - (float)d->top_left.x);
+                rel_y = (R)(float)(sp.y - (float)d->top_left.y);
+            }
+            R bw = (R)settings.s.band_width;
+            for (int i = 0; i < d->size.x; ++i)
+                for (int j = 0; j < d->size.y; ++j) {
+                    R ax = rel_x - (R)i, ay = rel_y - (R)j;
+                    R sq = ax * dx * ax * dx + ay * dx * ay * dx;
+                    R pressure = std::exp(-bw * sq);
+                    R angle = std::atan2(ax, ay);
+                    R c = std::cos(angle), s = std::sin(angle);
+                    R hc = c * c * pressure;
+                    R vc = s * s * pressure;
+                    d->current.p0(j, i) += pressure;
+                    d->current.px0(j, i) += hc;
+                    d->current.py0(j, i) += vc;
+                }
+        }
+        speakers.push_back(sp);
+    }
+
+    // Scene::add_receiver (Scene.cpp:254-265) + Receiver ctor (Receiver.cpp:34-44)
+    void add_receiver(float x, float y, float z, int id) {
+        float loc[3] = {x, y, z};
+        Domain<R> *container = nullptr;
+        for (Domain<R> *d : domain_list)
+            if (!d->is_pml && d->contains_location(loc)) container = d;
+        if (!container) throw std::runtime_error("receiver outside every domain");
+        Receiver<R> r;
+        std::memcpy(r.loc, loc, sizeof loc);
+        r.grid = Point((int)x, (int)y, (int)z);
+        r.container = container;
+        r.id = id;
+        receivers.push_back(r);
+    }
+
+    // Receiver::compute_with_nn (Receiver.cpp:80-83)
+    bool receiver_cell(const Receiver<R> &r, int *row, int *col) const {
+        Point rel = r.grid - r.container->top_left;
+        int ny = r.container->size.y, nx = r.container->size.x;
+        if (receiver_mode == PSTD_RECEIVER_REFERENCE) {
+            *row = rel.x;
+            *col = rel.y;
+        } else {
+            *row = rel.y;
+            *col = rel.x;
+        }
+        return *row >= 0 && *row < ny && *col >= 0 && *col < nx;
+    }
+
+    void compute_rk_step(int rk_step);
+    void compute_timestep(int frame, double *samples, oracle_frame_cb cb, void *user);
+};
+
+// Domain::create_attenuation_array (Domain.cpp:743-780)
+template <class R>
+void Domain<R>::create_attenuation_array(CalcDir cd, bool ascending, Mat<R> &pp, Mat<R> &pv) {
+    const Settings &st = scene->settings;
+    int N = st.s.pml_cells;
+    R ampmax = (R)st.s.ampmax, rho_air = (R)st.s.rho;
+    R dt = sizeof(R) == sizeof(float) ? (R)st.dt() : (R)st.s.tfactRK * (R)st.s.grid_spacing / (R)st.s.c1;
+    std::vector<R> fp(N), fv(N + 1);
+    R stepp = N > 1 ? ((R)(float)(N - 0.5) - (R)0.5) / (R)(N - 1) : 0;
+    for (int i = 0; i < N; ++i) {
+        R r = ((R)0.5 + (R)i * stepp) / (R)N;
+        R a = ampmax * std::pow(r, (R)4);
+        fp[i] = std::exp(-a * dt / rho_air);
+    }
+    R stepv = N > 0 ? ((R)N - (R)0) / (R)N : 0;
+    for (int i = 0; i <= N; ++i) {
+        R r = ((R)0 + (R)i * stepv) / (R)N;
+        R a = rho_air * ampmax * std::pow(r, (R)4);
+        fv[i] = std::exp(-a * dt);
+    }
+    if (!ascending) {
+        std::reverse(fp.begin(), fp.end());
+        std::reverse(fv.begin(), fv.end());
+    }
+    if (cd == X) {
+        if (size.x != N) throw std::runtime_error("X-attenuating PML width differs from PMLCells");
+        pp = Mat<R>(size.y, N);
+        pv = Mat<R>(size.y, N + 1);
+        for (int r = 0; r < size.y; ++r) {
+            for (int c = 0; c < N; ++c) pp(r, c) = fp[c];
+            for (int c = 0; c <= N; ++c) pv(r, c) = fv[c];
+        }
+    } else {
+        if (size.y != N) throw std::runtime_error("Y-attenuating PML height differs from PMLCells");
+        pp = Mat<R>(N, size.x);
+        pv = Mat<R>(N + 1, size.x);
+        for (int r = 0; r < N; ++r)
+            for (int c = 0; c < size.x; ++c) pp(r, c) = fp[r];
+        for (int r = 0; r <= N; ++r)
+            for (int c = 0; c < size.x; ++c) pv(r, c) = fv[r];
+    }
+}
+
+// Domain::compute_pml_matrices (Domain.cpp:644-714)
+template <class R>
+void Domain<R>::compute_pml_matrices() {
+    compute_attenuation_flags();
+    auto ones = [](int r, int c) {
+        Mat<R> m(r, c);
+        std::fill(m.a.begin(), m.a.end(), (R)1);
+        return m;
+    };
+    if (is_secondary_pml && is_corner_domain) {
+        create_attenuation_array(X, needs_reversed_attenuation.at(0), pml.px, pml.vx);
+        create_attenuation_array(Y, needs_reversed_attenuation.at(1), pml.py, pml.vy);
+    } else if (has_horizontal_attenuation) {
+        create_attenuation_array(X, needs_reversed_attenuation.at(0), pml.px, pml.vx);
+        pml.py = ones(size.y, size.x);
+        pml.vy = ones(size.y + 1, size.x);
+    } else {
+        create_attenuation_array(Y, needs_reversed_attenuation.at(0), pml.py, pml.vy);
+        pml.px = ones(size.y, size.x);
+        pml.vx = ones(size.y, size.x + 1);
+    }
+}
+
+// Domain::calc (Domain.cpp:95-391)
+template <class R>
+void Domain<R>::calc(CalcDir cd, CalcType ct) {
+    Scene<R> &sc = *scene;
+    std::vector<int> own = get_range(cd);
+    std::vector<Domain *> &d1s = cd == X ? left : bottom;
+    std::vector<Domain *> &d2s = cd == X ? right : top;
+    Mat<R> &source = ct == PRESSURE ? (cd == X ? L.Lpx : L.Lpy) : (cd == X ? L.Lvx : L.Lvy);
+    for (size_t i = 0; i != d1s.size() + 1; ++i) {
+        Domain *d1 = i != d1s.size() ? d1s[i] : nullptr;
+        for (size_t j = 0; j != d2s.size() + 1; ++j) {
+            Domain *d2 = j != d2s.size() ? d2s[j] : nullptr;
+            std::vector<int> inter = own;
+            if (d1) {
+                std::vector<int> r1 = d1->get_range(cd), t;
+                std::set_intersection(r1.begin(), r1.end(), inter.begin(), inter.end(), std::back_inserter(t));
+                inter = t;
+            }
+            if (d2) {
+                std::vector<int> r2 = d2->get_range(cd), t;
+                std::set_intersection(r2.begin(), r2.end(), inter.begin(), inter.end(), std::back_inserter(t));
+                inter = t;
+            }
+            if (inter.empty()) continue;
+            {
+                std::vector<int> diff;
+                std::set_difference(own.begin(), own.end(), inter.begin(), inter.end(),
+                                    std::inserter(diff, diff.begin()));
+                own = diff;
+            }
+            int rs = *std::min_element(inter.begin(), inter.end());
+            int re = *std::max_element(inter.begin(), inter.end()) + 1;
+            int full_range = re - rs;
+            int primary = cd == X ? size.x : size.y;
+            int result_dim = primary;
+            int wlen = sc.settings.window_size();
+            while (wlen > primary) wlen = wlen / 2;
+            int N_total = 2 * wlen + primary;
+            std::vector<R> wind = get_window_coefficients<R>(wlen, (int)sc.settings.s.patcherror);
+            if (ct == PRESSURE) {
+                N_total++;
+                result_dim++;
+            } else {
+                primary++;
+            }
+            Domain *s1 = d1, *s2 = d2;
+            Mat<R> side1, side2;
+            bool z1 = false, z2 = false;
+            if (!s1) {
+                s1 = this;
+                z1 = true;
+                side1 = ct == VELOCITY ? (cd == X ? extended_zeros(0, 1) : extended_zeros(1, 0)) : extended_zeros(0, 0);
+            }
+            if (!s2) {
+                s2 = this;
+                z2 = true;
+                side2 = ct == VELOCITY ? (cd == X ? extended_zeros(0, 1) : extended_zeros(1, 0)) : extended_zeros(0, 0);
+            }
+            const Mat<R> &main = ct == PRESSURE ? current.p0 : (cd == X ? current.vx0 : current.vy0);
+            if (!z1) side1 = ct == PRESSURE ? s1->current.p0 : (cd == X ? s1->current.vx0 : s1->current.vy0);
+            if (!z2) side2 = ct == PRESSURE ? s2->current.p0 : (cd == X ? s2->current.vx0 : s2->current.vy0);
+            const Discretization<R> &disc = sc.get_discretization(N_total);
+            const std::vector<std::complex<R>> &derfact = ct == PRESSURE ? disc.pressure : disc.velocity;
+            RhoArray<R> rho_array = get_rho_array<R>(s1->rho, rho, s2->rho);
+            Mat<R> res;
+            if (cd == X) {
+                int nrows = re - rs;
+                Mat<R> m = main.block(rs - top_left.y, 0, nrows, main.cols);
+                Mat<R> a = side1.block(rs - s1->top_left.y, 0, nrows, side1.cols);
+                Mat<R> b = side2.block(rs - s2->top_left.y, 0, nrows, side2.cols);
+                res = spatderp3<R>(a, m, b, derfact, rho_array, wind, wlen, ct, cd, sc.fftc);
+                if (res.rows != full_range || res.cols != result_dim) throw std::runtime_error("calc shape");
+                source.set_block(rs - top_left.y, 0, res);
+            } else {
+                int ncols = re - rs;
+                Mat<R> m = main.block(0, rs - top_left.x, main.rows, ncols);
+                Mat<R> a = side1.block(0, rs - s1->top_left.x, side1.rows, ncols);
+                Mat<R> b = side2.block(0, rs - s2->top_left.x, side2.rows, ncols);
+                res = spatderp3<R>(a, m, b, derfact, rho_array, wind, wlen, ct, cd, sc.fftc);
+                if (res.rows != result_dim || res.cols != full_range) throw std::runtime_error("calc shape");
+                source.set_block(0, rs - top_left.x, res);
+            }
+        }
+    }
+}
+
+// SingleThreadSolver / MultiThreadSolver::compute_rk_step (Solver.cpp:95-151)
+// and Solver::update_field_values (Solver.cpp:162-182).
+template <class R>
+void Scene<R>::compute_rk_step(int rk_step) {
+    struct Task {
+        Domain<R> *d;
+        CalcDir cd;
+        CalcType ct;
+    };
+    std::vector<Task> tasks;
+    for (CalcDir cd : {X, Y})
+        for (CalcType ct : {PRESSURE, VELOCITY})
+            for (Domain<R> *d : domain_list)
+                if (!d->is_rigid() && d->should_update[cd]) tasks.push_back({d, cd, ct});
+    if (threads > 1) {
+        std::string err;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+        for (long t = 0; t < (long)tasks.size(); ++t) {
+            try {
+                tasks[t].d->calc(tasks[t].cd, tasks[t].ct);
+            } catch (const std::exception &e) {
+#pragma omp critical
+                err = e.what();
+            }
+        }
+        if (!err.empty()) throw std::runtime_error(err);
+    } else {
+        for (auto &t : tasks) t.d->calc(t.cd, t.ct);
+    }
+    R dt = sizeof(R) == sizeof(float) ? (R)settings.dt()
+                                      : (R)settings.s.tfactRK * (R)settings.s.grid_spacing / (R)settings.s.c1;
+    R c1 = (R)settings.s.c1;
+    R c1sq = c1 * c1;
+    R coef = (R)settings.s.rk_coefficients[rk_step];
+    R s = dt * coef;
+    for (Domain<R> *d : domain_list) {
+        if (d->is_rigid()) continue;
+        R rho = d->rho;
+        auto upd = [&](Mat<R> &cur, const Mat<R> &prev, const Mat<R> &Lm, bool velocity_from_pressure) {
+            for (size_t k = 0; k < cur.a.size(); ++k) {
+                R l = velocity_from_pressure ? Lm.a[k] / rho : Lm.a[k] * rho * c1sq;
+                cur.a[k] = prev.a[k] - s * l;
+            }
+        };
+        upd(d->current.vx0, d->previous.vx0, d->L.Lpx, true);
+        upd(d->current.vy0, d->previous.vy0, d->L.Lpy, true);
+        upd(d->current.px0, d->previous.px0, d->L.Lvx, false);
+        upd(d->current.py0, d->previous.py0, d->L.Lvy, false);
+    }
+    for (Domain<R> *d : domain_list)
+        for (size_t k = 0; k < d->current.p0.a.size(); ++k)
+            d->current.p0.a[k] = d->current.px0.a[k] + d->current.py0.a[k];
+}
+
+// SingleThreadSolver::compute_timestep (Solver.cpp:71-93)
+template <class R>
+void Scene<R>::compute_timestep(int fr, double *samples, oracle_frame_cb cb, void *user) {
+    for (Domain<R> *d : domain_list) d->previous = d->current;
+    for (int k = 0; k < 6; ++k) compute_rk_step(k);
+    int save_nth = settings.s.save_nth;
+    for (Domain<R> *d : domain_list) {
+        if (save_nth > 0 && fr % save_nth == 0 && !d->is_pml && cb) {
+            std::vector<double> buf(d->current.p0.a.begin(), d->current.p0.a.end());
+            cb(user, fr, d->id, buf.data(), (long)buf.size());
+        }
+    }
+    for (Domain<R> *d : domain_list)
+        if (d->is_pml) {
+            for (size_t k = 0; k < d->current.px0.a.size(); ++k) d->current.px0.a[k] *= d->pml.px.a[k];
+            for (size_t k = 0; k < d->current.py0.a.size(); ++k) d->current.py0.a[k] *= d->pml.py.a[k];
+            for (size_t k = 0; k < d->current.vx0.a.size(); ++k) d->current.vx0.a[k] *= d->pml.vx.a[k];
+            for (size_t k = 0; k < d->current.vy0.a.size(); ++k) d->current.vy0.a[k] *= d->pml.vy.a[k];
+        }
+    for (size_t r = 0; r < receivers.size(); ++r) {
+        int row, col;
+        if (!receiver_cell(receivers[r], &row, &col)) throw std::runtime_error("receiver cell outside its domain");
+        R v = receivers[r].container->current.p0(row, col);
+        receivers[r].received.push_back(v);
+        if (samples) samples[r] = (double)v;
+    }
+}
+
+template <class R>
+static Scene<R> *build_scene(const pstd_config *conf, int threads) {
+    auto *sc = new Scene<R>();
+    try {
+        sc->settings.s = conf->settings;
+        sc->receiver_mode = conf->receiver_mode;
+        sc->threads = threads;
+        float dx = conf->settings.grid_spacing;
+        // PSTDKernel::add_domains (PSTDKernel.cpp:73-98)
+        std::vector<Domain<R> *> domains;
+        for (int i = 0; i < conf->n_domains; ++i) {
+            const pstd_domain_conf &dc = conf->domains[i];
+            float tlx = dc.top_left[0] / dx, tly = dc.top_left[1] / dx;
+            float sx = dc.size[0] / dx, sy = dc.size[1] / dx;
+            std::map<Direction, EdgeParameters> epm;
+            // translate_edge_parameters (PSTDKernel.cpp:181-190): kernel TOP <- conf.B, BOTTOM <- conf.T
+            epm[LEFT] = {dc.L.lr != 0, dc.L.absorption};
+            epm[RIGHT] = {dc.R.lr != 0, dc.R.absorption};
+            epm[TOP] = {dc.B.lr != 0, dc.B.absorption};
+            epm[BOTTOM] = {dc.T.lr != 0, dc.T.absorption};
+            int id = sc->get_new_id();
+            domains.push_back(
+                sc->make_domain(id, 1.f, Point((int)tlx, (int)tly), Point((int)sx, (int)sy), false, epm, nullptr));
+        }
+        for (auto *d : domains) sc->add_domain(d);
+        sc->add_pml_domains();
+        for (auto *d : sc->domain_list) {
+            d->compute_number_of_neighbours();
+            // find_update_directions reads the attenuation flags compute_pml_matrices
+            // sets; openPSTD reads them uninitialised (Domain.cpp:591-596 vs 652-684).
+            if (d->is_pml) d->compute_attenuation_flags();
+            d->find_update_directions();
+        }
+        // add_speakers / add_receivers (PSTDKernel.cpp:101-119)
+        for (int i = 0; i < conf->n_speakers; ++i) {
+            float x = conf->speakers[3 * i] / dx, y = conf->speakers[3 * i + 1] / dx;
+            sc->add_speaker(x, y, 0);
+        }
+        for (int i = 0; i < conf->n_receivers; ++i) {
+            float x = conf->receivers[3 * i] / dx, y = conf->receivers[3 * i + 1] / dx;
+            sc->add_receiver(x, y, 0, i);
+        }
+        // Scene::compute_pml_matrices (Scene.cpp:281-287)
+        for (auto *d : sc->domain_list)
+            if (d->is_pml) d->compute_pml_matrices();
+        // Wavenumber tables must cover the FFT length (quirk: they disagree when n+2w is a power of two).
+        for (auto *d : sc->domain_list) {
+            for (CalcDir cd : {X, Y}) {
+                int primary = cd == X ? d->size.x : d->size.y;
+                int w = sc->settings.window_size();
+                while (w > primary) w /= 2;
+                int Lp = 1 << match_number(2 * w + primary + 1), Fp = next_2_power_int(primary + 2 * w);
+                int Lv = 1 << match_number(2 * w + primary), Fv = next_2_power_int(primary + 1 + 2 * w);
+                if (Lp != Fp || Lv != Fv)
+                    throw std::runtime_error("domain size with n+2w a power of two: wavenumber table and FFT length disagree");
+            }
+        }
+        for (auto &r : sc->receivers) {
+            int row, col;
+            if (!sc->receiver_cell(r, &row, &col)) throw std::runtime_error("receiver cell outside its domain");
+        }
+    } catch (...) {
+        delete sc;
+        throw;
+    }
+    return sc;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+struct oracle_scene {
+    int precision = 64;
+    Scene<float> *s32 = nullptr;
+    Scene<double> *s64 = nullptr;
+};
+
+static std::string g_last_error;
+
+// Exported functions take C linkage from their declarations in pstd_oracle.h.
+
+int oracle_next_2_power_int(int n) { return next_2_power_int(n); }
+int oracle_next_2_power_float(float n) { return next_2_power_float(n); }
+
+void oracle_rho_array(double rho1, double rho_self, double rho2, int precision, double *out_p, double *out_v) {
+    if (precision == 32) {
+        auto r = get_rho_array<float>((float)rho1, (float)rho_self, (float)rho2);
+        for (int i = 0; i < 8; ++i) {
+            out_p[i] = r.p[i / 2][i % 2];
+            out_v[i] = r.v[i / 2][i % 2];
+        }
+    } else {
+        auto r = get_rho_array<double>(rho1, rho_self, rho2);
+        for (int i = 0; i < 8; ++i) {
+            out_p[i] = r.p[i / 2][i % 2];
+            out_v[i] = r.v[i / 2][i % 2];
+        }
+    }
+}
+
+void oracle_window(int w, int pe, int precision, double *out) {
+    if (precision == 32) {
+        auto v = get_window_coefficients<float>(w, pe);
+        for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
+    } else {
+        auto v = get_window_coefficients<double>(w, pe);
+        for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
+    }
+}
+
+template <class R>
+static int disc_impl(double dx, int n, double *wn, double *cf, double *pf, double *vf) {
+    auto d = discretize_wave_numbers<R>((R)dx, match_number(n));
+    int L = (int)d.wave_numbers.size();
+    for (int i = 0; i < L; ++i) {
+        if (wn) wn[i] = d.wave_numbers[i];
+        if (cf) { cf[2 * i] = d.complex_factors[i].real(); cf[2 * i + 1] = d.complex_factors[i].imag(); }
+        if (pf) { pf[2 * i] = d.pressure[i].real(); pf[2 * i + 1] = d.pressure[i].imag(); }
+        if (vf) { vf[2 * i] = d.velocity[i].real(); vf[2 * i + 1] = d.velocity[i].imag(); }
+    }
+    return L;
+}
+
+int oracle_discretization(double dx, int n, int precision, double *wn, double *cf, double *pf, double *vf) {
+    return precision == 32 ? disc_impl<float>(dx, n, wn, cf, pf, vf) : disc_impl<double>(dx, n, wn, cf, pf, vf);
+}
+
+template <class R>
+static long spat_impl(const double *p1, int r1, int c1, const double *p2, int r2, int c2, const double *p3, int r3,
+                      int c3, const double *df, int dfl, const double *rp, const double *rv, const double *win,
+                      int wlen, int ct, int dir, double *out, int *orow, int *ocol) {
+    auto tomat = [](const double *p, int r, int c) {
+        Mat<R> m(r, c);
+        for (size_t i = 0; i < m.a.size(); ++i) m.a[i] = (R)p[i];
+        return m;
+    };
+    std::vector<std::complex<R>> derfact(dfl);
+    for (int i = 0; i < dfl; ++i) derfact[i] = std::complex<R>((R)df[2 * i], (R)df[2 * i + 1]);
+    RhoArray<R> rho;
+    for (int i = 0; i < 8; ++i) {
+        rho.p[i / 2][i % 2] = (R)rp[i];
+        rho.v[i / 2][i % 2] = (R)rv[i];
+    }
+    std::vector<R> w(2 * wlen + 1);
+    for (int i = 0; i < 2 * wlen + 1; ++i) w[i] = (R)win[i];
+    FFTCache<R> fc;
+    Mat<R> res = spatderp3<R>(tomat(p1, r1, c1), tomat(p2, r2, c2), tomat(p3, r3, c3), derfact, rho, w, wlen,
+                              ct == 0 ? PRESSURE : VELOCITY, dir == 0 ? X : Y, fc);
+    *orow = res.rows;
+    *ocol = res.cols;
+    for (size_t i = 0; i < res.a.size(); ++i) out[i] = (double)res.a[i];
+    return (long)res.a.size();
+}
+
+long oracle_spatderp3(int precision, const double *p1, int r1, int c1, const double *p2, int r2, int c2,
+                      const double *p3, int r3, int c3, const double *df, int dfl, const double *rp, const double *rv,
+                      const double *win, int wlen, int ct, int dir, double *out, int *orow, int *ocol) {
+    try {
+        if (precision == 32)
+            return spat_impl<float>(p1, r1, c1, p2, r2, c2, p3, r3, c3, df, dfl, rp, rv, win, wlen, ct, dir, out,
+                                    orow, ocol);
+        return spat_impl<double>(p1, r1, c1, p2, r2, c2, p3, r3, c3, df, dfl, rp, rv, win, wlen, ct, dir, out, orow,
+                                 ocol);
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+
+template <class R>
+static void rfft_impl(int n, const double *x, double *spec) {
+    RealFFT<R> f(n);
+    std::vector<R> xr(n);
+    for (int i = 0; i < n; ++i) xr[i] = (R)x[i];
+    std::vector<std::complex<R>> X(n / 2 + 1), z;
+    f.forward(xr.data(), X.data(), z);
+    for (int k = 0; k <= n / 2; ++k) {
+        spec[2 * k] = X[k].real();
+        spec[2 * k + 1] = X[k].imag();
+    }
+}
+template <class R>
+static void irfft_impl(int n, const double *spec, double *y) {
+    RealFFT<R> f(n);
+    std::vector<std::complex<R>> X(n / 2 + 1), z;
+    for (int k = 0; k <= n / 2; ++k) X[k] = std::complex<R>((R)spec[2 * k], (R)spec[2 * k + 1]);
+    std::vector<R> yr(n);
+    f.inverse(X.data(), yr.data(), z);
+    for (int i = 0; i < n; ++i) y[i] = yr[i];
+}
+void oracle_rfft(int precision, int n, const double *x, double *spec) {
+    if (precision == 32) rfft_impl<float>(n, x, spec);
+    else rfft_impl<double>(n, x, spec);
+}
+void oracle_irfft(int precision, int n, const double *spec, double *y) {
+    if (precision == 32) irfft_impl<float>(n, spec, y);
+    else irfft_impl<double>(n, spec, y);
+}
+
+oracle_scene *oracle_create(const pstd_config *conf, int precision, int threads) {
+    try {
+        auto *o = new oracle_scene();
+        o->precision = precision;
+        if (precision == 32) o->s32 = build_scene<float>(conf, threads);
+        else o->s64 = build_scene<double>(conf, threads);
+        return o;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return nullptr;
+    }
+}
+const char *oracle_error(void) { return g_last_error.c_str(); }
+void oracle_destroy(oracle_scene *o) {
+    if (!o) return;
+    delete o->s32;
+    delete o->s64;
+    delete o;
+}
+
+#define DISPATCH(o, ...)             \
+    do {                             \
+        if ((o)->s32) {              \
+            auto &S = *(o)->s32;     \
+            __VA_ARGS__;             \
+        } else {                     \
+            auto &S = *(o)->s64;     \
+            __VA_ARGS__;             \
+        }                            \
+    } while (0)
+
+int oracle_num_steps(const oracle_scene *o) {
+    int n = 0;
+    DISPATCH(o, n = (int)(S.settings.s.calctime / S.settings.dt()));
+    return n;
+}
+int oracle_domain_count(const oracle_scene *o) {
+    int n = 0;
+    DISPATCH(o, n = (int)S.domain_list.size());
+    return n;
+}
+int oracle_receiver_count(const oracle_scene *o) {
+    int n = 0;
+    DISPATCH(o, n = (int)S.receivers.size());
+    return n;
+}
+int oracle_domain_info(const oracle_scene *o, int i, int32_t *info, double *finfo) {
+    int ok = -1;
+    DISPATCH(o, {
+        if (i >= 0 && i < (int)S.domain_list.size()) {
+            auto *d = S.domain_list[i];
+            int32_t v[16] = {d->id, d->top_left.x, d->top_left.y, d->size.x, d->size.y, d->is_pml,
+                             d->is_secondary_pml, d->is_rigid(), d->local, (int)d->left.size(), (int)d->right.size(),
+                             (int)d->top.size(), (int)d->bottom.size(), d->should_update[0], d->should_update[1],
+                             (int)d->pml_for.size()};
+            std::memcpy(info, v, sizeof v);
+            finfo[0] = d->alpha;
+            finfo[1] = (double)d->impedance_r;
+            finfo[2] = (double)d->rho;
+            ok = 0;
+        }
+    });
+    return ok;
+}
+
+long oracle_read_field(const oracle_scene *o, int i, int field, double *out) {
+    long n = -1;
+    DISPATCH(o, {
+        if (i >= 0 && i < (int)S.domain_list.size()) {
+            auto *d = S.domain_list[i];
+            auto *m = field == PSTD_FIELD_P0 ? &d->current.p0 : field == PSTD_FIELD_PX ? &d->current.px0
+                      : field == PSTD_FIELD_PY ? &d->current.py0 : field == PSTD_FIELD_VX ? &d->current.vx0
+                      : &d->current.vy0;
+            n = (long)m->a.size();
+            if (out)
+                for (long k = 0; k < n; ++k) out[k] = (double)m->a[k];
+        }
+    });
+    return n;
+}
+long oracle_write_field(oracle_scene *o, int i, int field, const double *in) {
+    long n = -1;
+    DISPATCH(o, {
+        if (i >= 0 && i < (int)S.domain_list.size()) {
+            auto *d = S.domain_list[i];
+            auto *m = field == PSTD_FIELD_P0 ? &d->current.p0 : field == PSTD_FIELD_PX ? &d->current.px0
+                      : field == PSTD_FIELD_PY ? &d->current.py0 : field == PSTD_FIELD_VX ? &d->current.vx0
+                      : &d->current.vy0;
+            n = (long)m->a.size();
+            using RT = typename std::remove_reference<decltype(m->a[0])>::type;
+            for (long k = 0; k < n; ++k) m->a[k] = (RT)in[k];
+        }
+    });
+    return n;
+}
+long oracle_read_pml(const oracle_scene *o, int i, int which, double *out) {
+    long n = -1;
+    DISPATCH(o, {
+        if (i >= 0 && i < (int)S.domain_list.size()) {
+            auto *d = S.domain_list[i];
+            auto *m = which == 0 ? &d->pml.px : which == 1 ? &d->pml.py : which == 2 ? &d->pml.vx : &d->pml.vy;
+            n = (long)m->a.size();
+            if (out)
+                for (long k = 0; k < n; ++k) out[k] = (double)m->a[k];
+        }
+    });
+    return n;
+}
+int oracle_neighbours(const oracle_scene *o, int i, int dir, int32_t *ids, int max) {
+    int n = -1;
+    DISPATCH(o, {
+        if (i >= 0 && i < (int)S.domain_list.size()) {
+            auto &v = S.domain_list[i]->neighbours_at((Direction)dir);
+            n = (int)v.size();
+            for (int k = 0; k < n && k < max; ++k) ids[k] = v[k]->id;
+        }
+    });
+    return n;
+}
+int oracle_receiver_cell(const oracle_scene *o, int r, int32_t *domain, int32_t *index) {
+    int ok = -1;
+    DISPATCH(o, {
+        if (r >= 0 && r < (int)S.receivers.size()) {
+            int row, col;
+            auto &rc = S.receivers[r];
+            int di = (int)(std::find(S.domain_list.begin(), S.domain_list.end(), rc.container) - S.domain_list.begin());
+            *domain = di;
+            *index = S.receiver_cell(rc, &row, &col) ? row * rc.container->size.x + col : -1;
+            ok = 0;
+        }
+    });
+    return ok;
+}
+int oracle_step(oracle_scene *o, int nsteps, double *samples, oracle_frame_cb cb, void *user) {
+    try {
+        DISPATCH(o, {
+            int nr = (int)S.receivers.size();
+            for (int k = 0; k < nsteps; ++k) {
+                S.compute_timestep(S.frame, samples ? samples + (size_t)k * nr : nullptr, cb, user);
+                S.frame++;
+            }
+        });
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+int oracle_rk_stage(oracle_scene *o, int k) {
+    try {
+        DISPATCH(o, S.compute_rk_step(k));
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+void oracle_push_values(oracle_scene *o) {
+    DISPATCH(o, for (auto *d : S.domain_list) d->previous = d->current);
+}
+void oracle_apply_pml(oracle_scene *o) {
+    DISPATCH(o, {
+        for (auto *d : S.domain_list)
+            if (d->is_pml) {
+                for (size_t k = 0; k < d->current.px0.a.size(); ++k) d->current.px0.a[k] *= d->pml.px.a[k];
+                for (size_t k = 0; k < d->current.py0.a.size(); ++k) d->current.py0.a[k] *= d->pml.py.a[k];
+                for (size_t k = 0; k < d->current.vx0.a.size(); ++k) d->current.vx0.a[k] *= d->pml.vx.a[k];
+                for (size_t k = 0; k < d->current.vy0.a.size(); ++k) d->current.vy0.a[k] *= d->pml.vy.a[k];
+            }
+    });
+}
+

@@ -1,0 +1,79 @@
+/*
+ * TEST INFRASTRUCTURE ONLY — the CPU oracle of openpstd_amd.
+ *
+ * A from-scratch C++ restatement of openPSTD's PSTD update loop (scene
+ * construction, windowed spectral line derivative, RK6 loop, PML), templated
+ * on float (FP32, the reference's arithmetic) and double (FP64, the same
+ * method in double precision).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  It is never part of the product
+ * path.
+ */
+#ifndef PSTD_ORACLE_H
+#define PSTD_ORACLE_H
+
+#include "../include/openpstd_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Known-answer primitives (kernel/core/kernel_functions.cpp, WisdomCache.cpp) */
+int oracle_next_2_power_int(int n);
+int oracle_next_2_power_float(float n);
+/* out_p / out_v: 8 values each, row-major 4x2 (kernel_functions.cpp:37-64) */
+void oracle_rho_array(double rho1, double rho_self, double rho2, int precision, double *out_p, double *out_v);
+/* 2w+1 window coefficients (kernel_functions.cpp:327-333) */
+void oracle_window(int w, int patch_error, int precision, double *out);
+/* wavenumber discretisation for N (WisdomCache.cpp:39-75): returns table length L;
+ * outputs (when non-NULL) have L entries; complex arrays interleaved re,im. */
+int oracle_discretization(double dx, int n, int precision, double *wave_numbers, double *complex_factors,
+                          double *pressure_factors, double *velocity_factors);
+/* spatderp3 on explicit matrices (kernel_functions.cpp:123-318).  Matrices are
+ * row-major rows x cols.  derfact is interleaved complex with derfact_len
+ * entries; rho4x2 is the 4x2 pressure or velocity stencil row-major;
+ * window has 2*wlen+1 entries.  direction 0 = X, 1 = Y; ct 0 = pressure,
+ * 1 = velocity.  Returns the number of output values written to out (row-major,
+ * out_rows x out_cols) or -1. */
+long oracle_spatderp3(int precision, const double *p1, int p1_rows, int p1_cols, const double *p2, int p2_rows,
+                      int p2_cols, const double *p3, int p3_rows, int p3_cols, const double *derfact,
+                      int derfact_len, const double *rho_pressure, const double *rho_velocity, const double *window,
+                      int wlen, int ct, int direction, double *out, int *out_rows, int *out_cols);
+/* Real FFT helpers of the oracle (exposed so tests can pin them against numpy). */
+void oracle_rfft(int precision, int n, const double *x, double *spectrum /* (n/2+1) complex interleaved */);
+void oracle_irfft(int precision, int n, const double *spectrum, double *y /* unnormalised c2r */);
+
+/* Whole-scene oracle. precision 32 or 64. threads: 1 = SingleThreadSolver
+ * ordering, >1 = MultiThreadSolver (OpenMP over calc tasks). */
+typedef struct oracle_scene oracle_scene;
+oracle_scene *oracle_create(const pstd_config *conf, int precision, int threads);
+const char *oracle_error(void);
+void oracle_destroy(oracle_scene *s);
+int oracle_num_steps(const oracle_scene *s);   /* (int)(calctime/dt) */
+int oracle_domain_count(const oracle_scene *s);
+int oracle_receiver_count(const oracle_scene *s);
+/* Same layout as pstd_kernel_domain_info. */
+int oracle_domain_info(const oracle_scene *s, int index, int32_t *info, double *finfo);
+/* field: PSTD_FIELD_*; returns element count. */
+long oracle_read_field(const oracle_scene *s, int index, int field, double *out);
+long oracle_write_field(oracle_scene *s, int index, int field, const double *in);
+/* PML factors 0 px, 1 py, 2 vx, 3 vy */
+long oracle_read_pml(const oracle_scene *s, int index, int which, double *out);
+/* Neighbour ids: dir 0 LEFT 1 RIGHT 2 TOP 3 BOTTOM; returns count. */
+int oracle_neighbours(const oracle_scene *s, int index, int dir, int32_t *ids, int max);
+/* Receiver container domain index and row-major p0 index (-1 when outside). */
+int oracle_receiver_cell(const oracle_scene *s, int r, int32_t *domain, int32_t *index);
+/* Advances nsteps time steps from the current frame.  samples: nsteps x
+ * n_receivers values (may be NULL).  frames_cb is called for every non-PML
+ * domain when frame % SaveNth == 0 (WriteFrame order of kernel/Solver.cpp:80-84). */
+typedef void (*oracle_frame_cb)(void *user, int frame, int domain_id, const double *data, long n);
+int oracle_step(oracle_scene *s, int nsteps, double *samples, oracle_frame_cb frames_cb, void *user);
+/* Runs one RK stage only (for stage-level parity tests): calc + update + p0. */
+int oracle_rk_stage(oracle_scene *s, int rk_step);
+/* push_values / apply_pml_matrices, exposed for stage-level tests. */
+void oracle_push_values(oracle_scene *s);
+void oracle_apply_pml(oracle_scene *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

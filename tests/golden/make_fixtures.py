@@ -1,0 +1,72 @@
+"""Writes the committed fixtures of tests/golden/.
+
+1. Known-answer data held by openPSTD's own unit tests, read as text from the
+   reference checkout (only where it is present; the fixtures are committed):
+     window_w32_pe70.json      test/Kernel/kernel_functions.cpp:147
+     kat_reference.json        next_2_power, get_rho_array, wavenumbers, Gaussian
+2. Long-run oracle series (FP64 CPU oracle), too slow to recompute in a test:
+     c1_receivers.npz, c2_receivers.npz
+
+    python tests/golden/make_fixtures.py [--kat] [--series]
+"""
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+sys.path.insert(0, ROOT)
+
+
+def kat():
+    src = open(os.path.join(REF, "test/Kernel/kernel_functions.cpp")).read()
+    m = re.search(r"window_verify << ([^;]*);", src)
+    vals = [float(v) for v in m.group(1).split(",")]
+    json.dump({"source": "test/Kernel/kernel_functions.cpp:147 (window_verify, w=32, patch error 70)",
+               "values": vals}, open(os.path.join(HERE, "window_w32_pe70.json"), "w"), indent=1)
+    wc = open(os.path.join(REF, "test/Kernel/WisdomCache.cpp")).read()
+    checks = re.findall(r"is_approx\(discr1\.(\w+)(?:\.(real|imag)\(\))?\.coeff\((\d+)\), (-?[\d.]+)\)", wc)
+    data = {
+        "next_2_power": {"source": "test/Kernel/kernel_functions.cpp:47-51",
+                         "cases": [[42, 64], [3.4, 4], [0.1, 1]]},
+        "rho_array": {"source": "test/Kernel/kernel_functions.cpp:53-74",
+                      "cases": [{"rho": [1e10, 1.2, 1.2], "pressure": [1, -1, 0, 0, 2, 0, 1, 1],
+                                 "velocity": [-1, 1, 0, 0, 0, 2, 1, 1]},
+                                {"rho": [1.2, 1.2, 1.2], "pressure": [0, 0, 0, 0, 1, 1, 1, 1],
+                                 "velocity": [0, 0, 0, 0, 1, 1, 1, 1]}]},
+        "discretization": {"source": "test/Kernel/WisdomCache.cpp:48-98", "dx": 0.2, "n": 115,
+                           "table_len": 128, "larger": [[178, 256], [227, 256]], "max_k": 15.8,
+                           "checks": [[name, part or "", int(idx), float(v)] for name, part, idx, v in checks]},
+        "gaussian": {"source": "test/Kernel/Domain.cpp:47-70, 199-214", "center_rc": [25, 20],
+                     "neighbour_value": 0.706947},
+        "scene_counts": {"source": "test/Kernel/Scene.cpp:75-123, test/Kernel/Domain.cpp:97-106,168-173",
+                         "reflecting_alpha01": {"domains": 9, "pml": 8, "secondary": 4, "pml_side": 50},
+                         "domain_test": {"domains": 9, "size_x": 250, "bottom_right_y": 300,
+                                         "neighbours_no_pml": 0, "neighbours_all": 4}},
+    }
+    json.dump(data, open(os.path.join(HERE, "kat_reference.json"), "w"), indent=1)
+    print("wrote KAT fixtures")
+
+
+def series():
+    from oracle import binding as ob
+    from openpstd_amd import workloads as W
+    for name, conf, steps in (("c1", W.c1(500), 500), ("c2", W.c2(0, 2000), 2000)):
+        out = {}
+        for prec in (64, 32):
+            o = ob.OracleScene(conf, prec, threads=os.cpu_count() or 1)
+            out[f"fp{prec}"] = o.step(steps)
+            print(name, prec, "done", flush=True)
+        np.savez(os.path.join(HERE, f"{name}_receivers.npz"), **out)
+
+
+if __name__ == "__main__":
+    args = sys.argv[1:] or ["--kat", "--series"]
+    if "--kat" in args:
+        kat()
+    if "--series" in args:
+        series()

@@ -1,0 +1,164 @@
+"""The product's host scene model (C++, libopenpstd_amd.so) against the oracle's
+restatement of openPSTD's scene construction: domain graph, PML factors,
+initial fields, update flags, receivers and the device plan.  CPU only."""
+import numpy as np
+import pytest
+
+import openpstd_amd as P
+from openpstd_amd import abi_types as T
+from openpstd_amd import workloads as W
+from oracle import binding as ob
+
+from test_oracle_kat import _domain_test_conf
+
+
+def _scenes():
+    yield "c1", W.c1(20)
+    for seed in range(4):
+        yield f"c2_seed{seed}", W.c2(seed, 20)
+    for i, c in enumerate(W.c4(1, 20, n_scenes=8, hi=260)):
+        yield f"c4_{i}", c
+    yield "default_conf", P.PSTDConfiguration.CreateDefaultConf()
+    yield "domain_test", _domain_test_conf()
+
+
+SCENES = list(_scenes())
+
+
+@pytest.mark.parametrize("name,conf", SCENES, ids=[s[0] for s in SCENES])
+def test_scene_matches_oracle(name, conf):
+    k = P.PSTDKernel(GPU=True)
+    k.initialize_kernel(conf, P.KernelCallbackLog())
+    o = ob.OracleScene(conf, 64)
+    o32 = ob.OracleScene(conf, 32)  # impedance and rho follow openPSTD's float arithmetic
+    n = k.domain_count()
+    assert n == o.n_domains
+    ids = [o.domain_info(i)["id"] for i in range(n)]
+    for i in range(n):
+        a, b = k.domain_info(i), o32.domain_info(i)
+        for key in a:
+            if key == "impedance" and a["is_rigid"]:
+                # 1/(1 - sqrt(1 - 1e-5)) cancels: float and double differ; both > 1000
+                assert a[key] > 1000 and b[key] > 1000
+            elif key in ("alpha", "impedance", "rho"):
+                assert a[key] == pytest.approx(b[key], rel=1e-7), (i, key)
+            else:
+                assert a[key] == b[key], (i, key)
+        for f in ("p0", "px0", "py0", "vx0", "vy0"):
+            x, y = k.field(i, f), o.field(i, f)
+            assert x.shape == y.shape
+            assert np.abs(x - y).max() <= 2e-7 * max(1.0, np.abs(y).max()), (i, f)
+        if a["is_pml"]:
+            for w in range(4):
+                assert np.abs(k.pml(i, w).ravel() - o.pml(i, w)).max() < 1e-6, (i, w)
+    meta = k.get_metadata()
+    assert meta.Framecount == o.num_steps == conf.num_steps()
+    assert meta.DomainMetadata == [[o.domain_info(i)["nx"], o.domain_info(i)["ny"], 0] for i in range(n)]
+    assert ids[0] == 0
+
+
+@pytest.mark.parametrize("name,conf", SCENES[:8], ids=[s[0] for s in SCENES[:8]])
+def test_segments_cover_every_line_once(name, conf):
+    """Domain::calc resolves each line of each (direction, field) to exactly one
+    neighbour pair (kernel/core/Domain.cpp:149-186)."""
+    k = P.PSTDKernel(GPU=True)
+    k.initialize_kernel(conf)
+    infos = [k.domain_info(i) for i in range(k.domain_count())]
+    cover = {}
+    for s in k.segments():
+        d = infos[s["domain"]]
+        nlines = d["ny"] if s["direction"] == 0 else d["nx"]
+        key = (s["domain"], s["direction"], s["field"])
+        c = cover.setdefault(key, np.zeros(nlines, int))
+        c[s["line_begin"]:s["line_end"]] += 1
+        for side, off in ((s["side1"], s["side1_offset"]), (s["side2"], s["side2_offset"])):
+            if side >= 0:
+                o = infos[side]
+                nl = o["ny"] if s["direction"] == 0 else o["nx"]
+                assert 0 <= s["line_begin"] + off and s["line_end"] + off <= nl
+        assert s["wlen"] == 32 and s["table_len"] >= 128
+    for i, d in enumerate(infos):
+        for cd in (0, 1):
+            upd = d["update_x"] if cd == 0 else d["update_y"]
+            for ct in (0, 1):
+                if d["is_rigid"] or not upd:
+                    assert (i, cd, ct) not in cover
+                else:
+                    assert (cover[(i, cd, ct)] == 1).all(), (i, cd, ct)
+
+
+def test_neighbour_stencils_match_rho_arrays():
+    """Segment coefficients are openPSTD's get_rho_array entries for the pair."""
+    conf = W.c2(2, 10)
+    k = P.PSTDKernel(GPU=True)
+    k.initialize_kernel(conf)
+    infos = [k.domain_info(i) for i in range(k.domain_count())]
+    for s in k.segments():
+        r_self = infos[s["domain"]]["rho"]
+        r1 = infos[s["side1"]]["rho"] if s["side1"] >= 0 else r_self
+        r2 = infos[s["side2"]]["rho"] if s["side2"] >= 0 else r_self
+        p, v = ob.rho_array(r1, r_self, r2, 64)
+        m = p if s["field"] == 0 else v
+        want = [m[2, 1], m[0, 0], m[3, 1], m[1, 0]]
+        got = [s["c_side1"], s["c_main1"], s["c_side2"], s["c_main2"]]
+        assert np.allclose(got, want, rtol=1e-6, atol=1e-30)
+
+
+def test_receiver_modes():
+    conf = W.c2(0, 10)
+    for mode in (T.PSTD_RECEIVER_PHYSICAL, T.PSTD_RECEIVER_REFERENCE):
+        conf.receiver_mode = mode
+        conf.Receivers = [(W._world_pos(40.25), W._world_pos(30.25), 0.0)]
+        k = P.PSTDKernel()
+        k.initialize_kernel(conf)
+        o = ob.OracleScene(conf, 32)
+        d, idx = o.receiver_cell(0)
+        assert d == 0
+        assert idx == (30 * 256 + 40 if mode == T.PSTD_RECEIVER_PHYSICAL else 40 * 256 + 30)
+    # openPSTD's transposed read of a cell outside the array is an error
+    conf.Receivers = [(W._world_pos(200.25), W._world_pos(30.25), 0.0)]
+    with pytest.raises(P.PSTDError, match="receiver"):
+        P.PSTDKernel().initialize_kernel(conf)
+    with pytest.raises(RuntimeError, match="receiver"):
+        ob.OracleScene(conf, 32)
+
+
+def test_receiver_outside_every_domain():
+    conf = W.c1(5)
+    conf.Receivers = [(500.0, 500.0, 0.0)]
+    with pytest.raises(P.PSTDError, match="outside"):
+        P.PSTDKernel().initialize_kernel(conf)
+
+
+def test_rejects_wavenumber_fft_mismatch():
+    conf = P.PSTDConfiguration()
+    W.base_settings(conf, 5)
+    conf.Domains = [W.domain(0, 0, 192, 100, 1.0)]
+    with pytest.raises(P.PSTDError, match="power of two"):
+        P.PSTDKernel().initialize_kernel(conf)
+
+
+def test_not_configured():
+    """run()/get_metadata() before initialize_kernel (kernel/PSTDKernel.cpp:122-123,155-156)."""
+    k = P.PSTDKernel(GPU=True)
+    with pytest.raises(P.PSTDKernelNotConfiguredException):
+        k.get_metadata()
+    with pytest.raises(P.PSTDKernelNotConfiguredException):
+        k.run(P.RecordingCallback())
+
+
+def test_cpu_solver_not_provided():
+    k = P.PSTDKernel(GPU=False)
+    k.initialize_kernel(W.c1(5))
+    with pytest.raises(P.PSTDError) as e:
+        k.run(P.RecordingCallback())
+    assert e.value.status == T.PSTD_ERR_UNSUPPORTED
+
+
+def test_settings_mirror():
+    s = P.PSTDConfiguration.CreateDefaultConf().Settings
+    assert s.GetWindowSize() == 32
+    assert s.GetTimeStep() == pytest.approx(0.5 * 0.2 / 340, rel=1e-6)
+    assert s.GetBandWidth() == pytest.approx(8.67, rel=1e-6)
+    c1 = W.c1()
+    assert c1.num_steps() == 500 and W.c2().num_steps() == 2000
