@@ -174,6 +174,13 @@ int pstd_ctx_advance_async(pstd_ctx *ctx, int steps);
 int pstd_ctx_synchronize(pstd_ctx *ctx);
 /* Current p0 of one domain, row-major (Solver::get_pressure_vector). */
 int pstd_ctx_read_frame(pstd_ctx *ctx, int domain, float *out, size_t count);
+/* Advances `steps` time steps like pstd_ctx_advance_async, with a HIP event
+ * pair on the solver stream around every RK-stage kernel launch; returns the
+ * mean GPU duration of one stage launch (ms) and the number of launches timed.
+ * For measurement; the state advances exactly as with pstd_ctx_advance. */
+int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int *launches);
+/* Cells updated per RK stage: sum of nx*ny over the non-rigid domains. */
+long long pstd_ctx_cells_per_stage(const pstd_ctx *ctx);
 /* Number of time steps advanced since upload. */
 int pstd_ctx_steps_done(const pstd_ctx *ctx);
 const char *pstd_ctx_last_error(const pstd_ctx *ctx);

@@ -51,36 +51,34 @@ struct DomainDev {
     int is_pml;
 };
 
-struct SegDev {
-    int domain, dir, field;
-    int side1, side2;
-    int off1, off2;
-    int wlen;
-    int n_main;    // main line length (n for pressure, n+1 for velocity)
-    int n_out;     // derivative values per line (n+1 pressure, n velocity)
+// One work item of a stage launch; everything a workgroup needs in one
+// scalar-loadable record.
+//  kind 0: derivative tile — `count` lines of a segment (Domain::calc,
+//          kernel/core/Domain.cpp:149-370): gather, derivative, update.
+//  kind 1: passthrough — elements [line0, line0+count) of one field that no
+//          segment updates (rigid domains, directions openPSTD skips).
+// Source value of a domain at (line, pos):  A[a + line*ls + pos*ps]
+// (+ B[b + same] for p0 = px0 + py0).
+struct TileDesc {
+    int kind;
     int F;
-    int h_off;     // float2 offset of the derivative filter in the table buffer
-    int tw_off;    // float2 offset of the twiddle table
-    int win_off;   // float offset of the 2w+1 window
+    int line0, count;
+    int wlen, n_main, n_out, vel, dir;
+    int h_off, tw_off, win_off;
+    int m_a, m_b, m_ls, m_ps;
+    int s1_a, s1_b, s1_ls, s1_ps, s1_line, s1_len;  // s1_len < 0: no neighbour (zeros)
+    int s2_a, s2_b, s2_ls, s2_ps, s2_line, s2_len;
     float c_s1, c_m1, c_s2, c_m2;
-};
-
-struct TileDev {
-    int seg;    // derivative segment, or -1 for an elementwise passthrough tile
-    int line0;  // first local line (derivative) / first element (passthrough)
-    int count;  // lines (derivative) / elements (passthrough)
-    int domain; // passthrough: domain
-    int field;  // passthrough: PSTD_FIELD_*
+    int o_off, o_ls, o_ps;  // output field (vx, px, vy or py of the domain)
+    float rho;
+    int is_pml;
 };
 
 struct StageArgs {
-    const DomainDev *doms;
-    const SegDev *segs;
-    const TileDev *tiles;
-    int ntiles;
+    const TileDesc *tiles;
     const float2 *ctab;
     const float *ftab;
-    const float *p0a;  // p0 = p0a[px slot] + (p0b ? p0b[py slot] : 0)
+    const float *p0a;  // p0 = p0a[a + idx] + (p0b ? p0b[b + idx] : 0)
     const float *p0b;
     const float *vsrc; // current vx / vy
     float *tn;
@@ -93,152 +91,253 @@ struct StageArgs {
 
 // ------------------------------------------------------------- device code
 
-__device__ __forceinline__ int field_offset(const DomainDev &d, int f) {
-    return f == PSTD_FIELD_VX ? d.off_vx : f == PSTD_FIELD_VY ? d.off_vy : f == PSTD_FIELD_PX ? d.off_px : d.off_py;
+// Per-tile geometry shared by the generic (F-independent) parts of a tile.
+struct TileGeom {
+    int F, T, P, logT, logP, layout;
+};
+
+__device__ __forceinline__ TileGeom tile_geom(int F, int NT, int layout) {
+    TileGeom g;
+    g.F = F;
+    g.T = F >> 3;
+    g.P = NT / g.T;
+    g.logT = __ffs(g.T) - 1;
+    g.logP = __ffs(g.P) - 1;
+    g.layout = layout;
+    return g;
 }
 
-// Value of a domain's derivative source at (line, pos): p0 for pressure
-// segments, vx (dir X) / vy (dir Y) for velocity segments.
-__device__ __forceinline__ float src_value(const StageArgs &a, const DomainDev &d, int dir, int field, int line,
-                                           int pos) {
-    if (field == 0) {
-        int idx = dir == 0 ? line * d.nx + pos : pos * d.nx + line;
-        float v = a.p0a[d.off_px + idx];
-        if (a.p0b) v += a.p0b[d.off_py + idx];
-        return v;
-    }
-    if (dir == 0) return a.vsrc[d.off_vx + line * (d.nx + 1) + pos];
-    return a.vsrc[d.off_vy + pos * d.nx + line];
+// The windowed lines of one tile (spatderp3, kernel/core/kernel_functions.cpp:
+// 210-216 pressure, :263-269 velocity):  [dom1 | main | dom3 | 0], where
+// dom1[j] = (side1[tail + j] * c_s1 + main[reflected] * c_m1) * window[j] and
+// dom3 likewise.  Thread (p, t) loads exactly the raw values it will transform,
+// positions t + m*T of lines 2p and 2p+1, branch-free (every load issued from a
+// valid address, masked afterwards) so all its loads are in flight together.
+// The raw values go through LDS only so that halo positions can read the
+// reflected main values staged by other threads.
+__device__ __forceinline__ int stage_at(const TileGeom &g, int ll, int pos) {
+    return g.layout == 0 ? (ll << (g.logT + 3)) + pos : (pos << (g.logP + 1)) + ll;
 }
 
-// Element i of the windowed line [dom1 | main | dom3 | 0] (spatderp3,
-// kernel/core/kernel_functions.cpp:210-216 pressure, :263-269 velocity).
-__device__ __forceinline__ float line_value(const StageArgs &a, const SegDev &sg, const DomainDev &dm, int line,
-                                            int i) {
-    const int w = sg.wlen;
-    const int nm = sg.n_main;
-    if (i >= w && i < w + nm) return src_value(a, dm, sg.dir, sg.field, line, i - w);
-    const bool vel = sg.field == 1;
-    if (i < w) {
-        // a missing neighbour is a zero block with transparent coefficients: 0
-        if (sg.side1 < 0) return 0.f;
-        const int j = i;
-        const DomainDev &d1 = a.doms[sg.side1];
-        const int s1len = (sg.dir == 0 ? d1.nx : d1.ny) + (vel ? 1 : 0);
-        float side = src_value(a, d1, sg.dir, sg.field, line + sg.off1, vel ? s1len - w - 1 + j : s1len - w + j);
-        float main = src_value(a, dm, sg.dir, sg.field, line, vel ? w - j : w - 1 - j);
-        return (side * sg.c_s1 + main * sg.c_m1) * a.ftab[sg.win_off + j];
-    }
-    if (i < 2 * w + nm) {
-        const int j = i - w - nm;
-        if (sg.side2 < 0) return 0.f;
-        const DomainDev &d2 = a.doms[sg.side2];
-        float main = src_value(a, dm, sg.dir, sg.field, line, vel ? nm - 2 - j : nm - 1 - j);
-        float side = src_value(a, d2, sg.dir, sg.field, line + sg.off2, vel ? 1 + j : j);
-        return (side * sg.c_s2 + main * sg.c_m2) * a.ftab[sg.win_off + w + 1 + j];
-    }
-    return 0.f;
-}
-
-// Solver::update_field_values (kernel/Solver.cpp:174-181) for one value.
-__device__ __forceinline__ void write_update(const StageArgs &a, const SegDev &sg, const DomainDev &dm, int line,
-                                             int pos, float L) {
-    int idx;
-    if (sg.dir == 0) idx = sg.field == 0 ? dm.off_vx + line * (dm.nx + 1) + pos : dm.off_px + line * dm.nx + pos;
-    else idx = sg.field == 0 ? dm.off_vy + pos * dm.nx + line : dm.off_py + pos * dm.nx + line;
-    float upd = sg.field == 0 ? L / dm.rho : L * dm.rho * a.c2;
-    float v = a.tn[idx] - a.s * upd;
-    a.out[idx] = v;
-    if (a.last) a.tn[idx] = dm.is_pml ? v * a.pml[idx] : v;
-}
-
-template <int F, int LAYOUT>
-__device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDev &tile, float2 *buf0, float2 *buf1) {
-    constexpr int T = F / 8;
-    constexpr int NT = F >= 4096 ? F / 8 : kSmallThreads;
-    constexpr int P = NT / T;  // line pairs per block
-    const int tid = threadIdx.x;
-    const int p = LAYOUT == 0 ? tid / T : tid % P;
-    const int t = LAYOUT == 0 ? tid % T : tid / P;
-    const SegDev sg = a.segs[tile.seg];
-    const DomainDev dm = a.doms[sg.domain];
-    const int l0 = tile.line0 + 2 * p, l1 = l0 + 1;
-    const bool ok0 = 2 * p < tile.count, ok1 = 2 * p + 1 < tile.count;
-    float2 v[8];
+__device__ __forceinline__ void gather_lines(const StageArgs &a, const TileDesc &td, const TileGeom &g, int p, int t,
+                                             float *stage, float2 (&v)[8]) {
+    const int w = td.wlen, nm = td.n_main, vel = td.vel, count = td.count;
+    const int m_a = td.m_a, m_b = td.m_b, m_ls = td.m_ls, m_ps = td.m_ps;
+    const int s1_a = td.s1_a, s1_b = td.s1_b, s1_ls = td.s1_ls, s1_ps = td.s1_ps, s1_line = td.s1_line;
+    const int s2_a = td.s2_a, s2_b = td.s2_b, s2_ls = td.s2_ls, s2_ps = td.s2_ps, s2_line = td.s2_line;
+    const int s1_len = td.s1_len, s2_len = td.s2_len, win_off = td.win_off;
+    const float c_s1 = td.c_s1, c_m1 = td.c_m1, c_s2 = td.c_s2, c_m2 = td.c_m2;
+    const float *A = vel ? a.vsrc : a.p0a;
+    const bool hasB = !vel && a.p0b != nullptr;
+    const float *B = hasB ? a.p0b : A;
+    const int line_a = td.line0 + 2 * p;
+    const bool ok0 = 2 * p < count, ok1 = 2 * p + 1 < count;
+    float ra[8][2], rb[8][2], win[8];
+    bool okv[8][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        const int i = t + m * T;
-        v[m].x = ok0 ? line_value(a, sg, dm, l0, i) : 0.f;
-        v[m].y = ok1 ? line_value(a, sg, dm, l1, i) : 0.f;
+        const int pos = t + (m << g.logT);
+        const bool left = pos < w, right = pos >= w + nm, mid = !left && !right;
+        const int ba = left ? s1_a : (right ? s2_a : m_a);
+        const int bb = left ? s1_b : (right ? s2_b : m_b);
+        const int ls = left ? s1_ls : (right ? s2_ls : m_ls);
+        const int ps = left ? s1_ps : (right ? s2_ps : m_ps);
+        const int lo = left ? s1_line : (right ? s2_line : 0);
+        const int sp = left ? s1_len - w - vel + pos : (right ? pos - w - nm + vel : pos - w);
+        const bool vr = mid || (left ? s1_len >= 0 : (s2_len >= 0 && pos < 2 * w + nm));
+        const int i0 = (line_a + lo) * ls + sp * ps;
+        const bool v0 = vr && ok0, v1 = vr && ok1;
+        const int x0 = v0 ? ba + i0 : m_a, x1 = v1 ? ba + i0 + ls : m_a;
+        const int y0 = v0 && hasB ? bb + i0 : m_a, y1 = v1 && hasB ? bb + i0 + ls : m_a;
+        ra[m][0] = A[x0];
+        ra[m][1] = A[x1];
+        rb[m][0] = B[y0];
+        rb[m][1] = B[y1];
+        okv[m][0] = v0;
+        okv[m][1] = v1;
+        win[m] = a.ftab[win_off + (left ? pos : ((right && pos < 2 * w + nm) ? pos - nm + 1 : 0))];
     }
-    const float2 *tw = a.ctab + sg.tw_off;
-    fft_run<F, P, LAYOUT, false>(v, p, t, buf0, buf1, tw);
-    const float2 *H = a.ctab + sg.h_off;
+    float raw[8][2];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], H[t + m * T]);
+    for (int m = 0; m < 8; ++m) {
+        const int pos = t + (m << g.logT);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            raw[m][q] = okv[m][q] ? (hasB ? ra[m][q] + rb[m][q] : ra[m][q]) : 0.f;
+            stage[stage_at(g, 2 * p + q, pos)] = raw[m][q];
+        }
+    }
     __syncthreads();
-    fft_run<F, P, LAYOUT, true>(v, p, t, buf0, buf1, tw);
-    const int w = sg.wlen;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        const int i = t + m * T;
-        if (i >= w && i < w + sg.n_out) {
-            if (ok0) write_update(a, sg, dm, l0, i - w, v[m].x);
-            if (ok1) write_update(a, sg, dm, l1, i - w, v[m].y);
+        const int i = t + (m << g.logT);
+        const bool left = i < w, right = i >= w + nm && i < 2 * w + nm, mid = i >= w && i < w + nm;
+        const int j = i - w - nm;
+        const int mpos = left ? (vel ? w - i : w - 1 - i) : (right ? (vel ? nm - 2 - j : nm - 1 - j) : 0);
+        const float cs = left ? c_s1 : c_s2, cm = left ? c_m1 : c_m2;
+        float out[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const float refl = stage[stage_at(g, 2 * p + q, w + mpos)];
+            const float r = raw[m][q];
+            out[q] = mid ? r : ((left || right) ? (r * cs + refl * cm) * win[m] : 0.f);
+        }
+        v[m] = make_float2(out[0], out[1]);
+    }
+}
+
+// Forward transform, spectral derivative (the Hermitian filter h, which carries
+// the 1/F of kernel/core/kernel_functions.cpp:243,306), inverse transform.
+template <int F, int NT, int LAYOUT>
+__device__ __forceinline__ void derivative_fft(float2 (&v)[8], const float2 (&h)[8], const float2 *twr, int p, int t,
+                                               float2 *buf0, float2 *buf1) {
+    constexpr int P = NT / (F / 8);
+    fft_run<F, P, LAYOUT, false>(v, p, t, buf0, buf1, twr);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], h[m]);
+    __syncthreads();
+    fft_run<F, P, LAYOUT, true>(v, p, t, buf0, buf1, twr);
+}
+
+constexpr int kMaxTw = 28;  // fft_tw_total(F) for F <= 8192
+
+template <int F>
+__device__ __forceinline__ void prefetch_twiddles(float2 *twr, int t, const float2 *tw) {
+    static_assert(fft_tw_total(F) <= kMaxTw, "twiddle registers");
+    fft_load_twiddles<F>(twr, t, tw);
+}
+
+// Solver::update_field_values (kernel/Solver.cpp:174-181): branch-free loads of
+// the t_n values (and the PML factors in the last stage), then the stores.
+__device__ __forceinline__ void write_updates(const StageArgs &a, const TileDesc &td, const TileGeom &g,
+                                              const float2 (&v)[8], int p, int t) {
+    const int w = td.wlen, n_out = td.n_out, o_off = td.o_off, o_ls = td.o_ls, o_ps = td.o_ps;
+    const bool ok0 = 2 * p < td.count, ok1 = 2 * p + 1 < td.count;
+    const int base = o_off + (td.line0 + 2 * p) * o_ls;
+    const float rho = td.rho, c2 = a.c2, s = a.s;
+    const bool vel = td.vel != 0;
+    const bool use_pml = a.last && td.is_pml;
+    int idx[8][2];
+    float tnv[8][2], pm[8][2];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int pos = t + (m << g.logT) - w;
+        const bool okp = pos >= 0 && pos < n_out;
+        const int i0 = base + pos * o_ps;
+        idx[m][0] = okp && ok0 ? i0 : -1;
+        idx[m][1] = okp && ok1 ? i0 + o_ls : -1;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int id = idx[m][q] >= 0 ? idx[m][q] : o_off;
+            tnv[m][q] = a.tn[id];
+            pm[m][q] = use_pml ? a.pml[id] : 1.f;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int id = idx[m][q];
+            const float L = q ? v[m].y : v[m].x;
+            const float upd = vel ? L * rho * c2 : L / rho;
+            const float nv = tnv[m][q] - s * upd;
+            if (id >= 0) {
+                a.out[id] = nv;
+                if (a.last) a.tn[id] = nv * pm[m][q];
+            }
         }
     }
 }
 
-__device__ void passthrough_tile(const StageArgs &a, const TileDev &tile) {
-    const DomainDev dm = a.doms[tile.domain];
-    const int base = field_offset(dm, tile.field);
-    for (int e = threadIdx.x; e < tile.count; e += blockDim.x) {
-        const int idx = base + tile.line0 + e;
-        float v = a.tn[idx];
-        a.out[idx] = v;
-        if (a.last && dm.is_pml) a.tn[idx] = v * a.pml[idx];
+// One derivative tile: NT threads, F = td.F (8 values per thread and line pair).
+template <int NT, int FMIN, int FMAX>
+__device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDesc &td, float2 *buf0, float2 *buf1) {
+    const TileGeom g = tile_geom(td.F, NT, td.dir);
+    const int tid = threadIdx.x;
+    const int p = g.layout == 0 ? tid >> g.logT : tid & (g.P - 1);
+    const int t = g.layout == 0 ? tid & (g.T - 1) : tid >> g.logP;
+    // data-independent loads first: filter and twiddles
+    float2 h[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) h[m] = a.ctab[td.h_off + t + m * g.T];
+    float2 twr[kMaxTw];
+    const float2 *tw = a.ctab + td.tw_off;
+    switch (td.F) {
+        case 128: if (FMIN <= 128 && FMAX >= 128) prefetch_twiddles<128>(twr, t, tw); break;
+        case 256: if (FMIN <= 256 && FMAX >= 256) prefetch_twiddles<256>(twr, t, tw); break;
+        case 512: if (FMIN <= 512 && FMAX >= 512) prefetch_twiddles<512>(twr, t, tw); break;
+        case 1024: if (FMIN <= 1024 && FMAX >= 1024) prefetch_twiddles<1024>(twr, t, tw); break;
+        case 2048: if (FMIN <= 2048 && FMAX >= 2048) prefetch_twiddles<2048>(twr, t, tw); break;
+        case 4096: if (FMIN <= 4096 && FMAX >= 4096) prefetch_twiddles<4096>(twr, t, tw); break;
+        case 8192: if (FMIN <= 8192 && FMAX >= 8192) prefetch_twiddles<8192>(twr, t, tw); break;
+        default: break;
+    }
+    float *stage = reinterpret_cast<float *>(buf1);
+    float2 v[8];
+    gather_lines(a, td, g, p, t, stage, v);
+#define PSTD_FFT_CASE(FV)                                                                           \
+    case FV:                                                                                        \
+        if constexpr (FMIN <= FV && FMAX >= FV) {                                                   \
+            if (g.layout == 0) derivative_fft<FV, NT, 0>(v, h, twr, p, t, buf0, buf1);              \
+            else derivative_fft<FV, NT, 1>(v, h, twr, p, t, buf0, buf1);                            \
+        }                                                                                           \
+        break;
+    switch (td.F) {
+        PSTD_FFT_CASE(128)
+        PSTD_FFT_CASE(256)
+        PSTD_FFT_CASE(512)
+        PSTD_FFT_CASE(1024)
+        PSTD_FFT_CASE(2048)
+        PSTD_FFT_CASE(4096)
+        PSTD_FFT_CASE(8192)
+        default: break;
+    }
+#undef PSTD_FFT_CASE
+    write_updates(a, td, g, v, p, t);
+}
+
+// Fields no segment updates keep their t_n value (openPSTD leaves their L at 0,
+// kernel/Solver.cpp:174-181); the last stage applies the PML factor.  Each
+// thread issues all of its loads before its stores.
+__device__ void passthrough_tile(const StageArgs &a, const TileDesc &td) {
+    constexpr int PER = kPassChunk / kSmallThreads;
+    float v[PER], f[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + k * kSmallThreads;
+        const bool ok = e < td.count;
+        const int idx = td.o_off + td.line0 + e;
+        v[k] = ok ? a.tn[idx] : 0.f;
+        f[k] = (ok && a.last && td.is_pml) ? a.pml[idx] : 1.f;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + k * kSmallThreads;
+        if (e >= td.count) continue;
+        const int idx = td.o_off + td.line0 + e;
+        a.out[idx] = v[k];
+        if (a.last && td.is_pml) a.tn[idx] = v[k] * f[k];
     }
 }
 
-template <int F>
-__device__ __forceinline__ void run_tile_F(const StageArgs &a, const TileDev &tile, float2 *lds) {
-    constexpr int NT = F >= 4096 ? F / 8 : kSmallThreads;
-    constexpr int P = NT / (F / 8);
-    const SegDev &sg = a.segs[tile.seg];
-    if (sg.dir == 0) {
-        using Lm = LdsMap<F, P, 0>;
-        derivative_tile<F, 0>(a, tile, lds, lds + Lm::SIZE);
-    } else {
-        using Lm = LdsMap<F, P, 1>;
-        derivative_tile<F, 1>(a, tile, lds, lds + Lm::SIZE);
-    }
-}
-
-// two LDS buffers of P * F = 8 * kSmallThreads complex values
+// two LDS buffers of P * F = 8 * NT complex values
 constexpr int lds_small_floats2() { return 2 * 8 * kSmallThreads; }
 
 __global__ void __launch_bounds__(kSmallThreads) stage_small_kernel(StageArgs a) {
     __shared__ float2 lds[lds_small_floats2()];
-    const TileDev tile = a.tiles[blockIdx.x];
-    if (tile.seg < 0) {
-        passthrough_tile(a, tile);
+    const TileDesc td = a.tiles[blockIdx.x];
+    if (td.kind == 1) {
+        passthrough_tile(a, td);
         return;
     }
-    switch (a.segs[tile.seg].F) {
-        case 128: run_tile_F<128>(a, tile, lds); break;
-        case 256: run_tile_F<256>(a, tile, lds); break;
-        case 512: run_tile_F<512>(a, tile, lds); break;
-        case 1024: run_tile_F<1024>(a, tile, lds); break;
-        case 2048: run_tile_F<2048>(a, tile, lds); break;
-        default: break;
-    }
+    derivative_tile<kSmallThreads, 128, 2048>(a, td, lds, lds + 8 * kSmallThreads);
 }
 
 template <int F>
 __global__ void __launch_bounds__(F / 8) stage_big_kernel(StageArgs a) {
     extern __shared__ float2 lds_dyn[];
-    const TileDev tile = a.tiles[blockIdx.x];
-    run_tile_F<F>(a, tile, lds_dyn);
+    const TileDesc td = a.tiles[blockIdx.x];
+    derivative_tile<F / 8, F, F>(a, td, lds_dyn, lds_dyn + F);
 }
 
 __global__ void receiver_kernel(const float *__restrict__ arena, const int2 *__restrict__ cells, int nrec,
@@ -286,9 +385,8 @@ struct pstd_ctx {
     bool planned = false;
     // plan
     std::vector<DomainDev> doms;
-    std::vector<SegDev> segs;
     std::vector<pstd_plan_domain> plan_domains;
-    std::vector<TileDev> tiles_small, tiles_4096, tiles_8192;
+    std::vector<TileDesc> tiles_small, tiles_4096, tiles_8192;
     int arena_floats = 0;
     float dt = 0, c2 = 0;
     float alpha[6] = {0};
@@ -298,9 +396,7 @@ struct pstd_ctx {
     float *pml = nullptr;
     float2 *ctab = nullptr;
     float *ftab = nullptr;
-    DomainDev *d_doms = nullptr;
-    SegDev *d_segs = nullptr;
-    TileDev *d_tiles = nullptr;  // small | 4096 | 8192
+    TileDesc *d_tiles = nullptr;  // small | 4096 | 8192
     int2 *d_cells = nullptr;
     int4 *d_ranges = nullptr;
     int n_ranges = 0;
@@ -318,16 +414,12 @@ struct pstd_ctx {
         (void)hipFree(pml);
         (void)hipFree(ctab);
         (void)hipFree(ftab);
-        (void)hipFree(d_doms);
-        (void)hipFree(d_segs);
         (void)hipFree(d_tiles);
         (void)hipFree(d_cells);
         (void)hipFree(d_ranges);
         (void)hipFree(d_samples);
         arena = pml = ftab = d_samples = nullptr;
         ctab = nullptr;
-        d_doms = nullptr;
-        d_segs = nullptr;
         d_tiles = nullptr;
         d_cells = nullptr;
         d_ranges = nullptr;
@@ -385,8 +477,6 @@ void build_window(int w, int patch_error, std::vector<float> &out) {
 
 int launch_stage(pstd_ctx *ctx, int k, bool first_of_step) {
     StageArgs a{};
-    a.doms = ctx->d_doms;
-    a.segs = ctx->d_segs;
     a.ctab = ctx->ctab;
     a.ftab = ctx->ftab;
     const int in = (k + 1) & 1, outi = k & 1;
@@ -414,18 +504,15 @@ int launch_stage(pstd_ctx *ctx, int k, bool first_of_step) {
     const int ns = (int)ctx->tiles_small.size(), n4 = (int)ctx->tiles_4096.size(), n8 = (int)ctx->tiles_8192.size();
     if (ns) {
         a.tiles = ctx->d_tiles;
-        a.ntiles = ns;
         hipLaunchKernelGGL(stage_small_kernel, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
     }
     if (n4) {
         a.tiles = ctx->d_tiles + ns;
-        a.ntiles = n4;
         size_t lds = 2 * sizeof(float2) * LdsMap<4096, 1, 0>::SIZE;
         hipLaunchKernelGGL(stage_big_kernel<4096>, dim3(n4), dim3(512), lds, ctx->stream, a);
     }
     if (n8) {
         a.tiles = ctx->d_tiles + ns + n4;
-        a.ntiles = n8;
         size_t lds = 2 * sizeof(float2) * LdsMap<8192, 1, 0>::SIZE;
         hipLaunchKernelGGL(stage_big_kernel<8192>, dim3(n8), dim3(1024), lds, ctx->stream, a);
     }
@@ -514,7 +601,6 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->release();
     ctx->doms.clear();
-    ctx->segs.clear();
     ctx->tiles_small.clear();
     ctx->tiles_4096.clear();
     ctx->tiles_8192.clear();
@@ -541,6 +627,10 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         off = align(off + p.ny * p.nx);
         d.off_py = off;
         off = align(off + p.ny * p.nx);
+        if (off < 0 || off > (1 << 30)) {
+            ctx->err = "scene too large for 32-bit arena offsets";
+            return PSTD_ERR_UNSUPPORTED;
+        }
         ctx->doms.push_back(d);
     }
     ctx->arena_floats = std::max(off, 64);
@@ -554,10 +644,31 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     std::map<int, int> tw_cache;
     std::map<int, int> win_cache;
     const double dx = plan->dx;
+    // source addressing of a domain's derivative input for (direction, field):
+    // pressure reads p0 = px + py (row stride nx), velocity vx (X, stride nx+1) or vy (Y)
+    auto src_addr = [&](const DomainDev &d, int dir, int field, int &a, int &b, int &ls, int &ps) {
+        if (field == 0) {
+            a = d.off_px;
+            b = d.off_py;
+            ls = dir == 0 ? d.nx : 1;
+            ps = dir == 0 ? 1 : d.nx;
+        } else if (dir == 0) {
+            a = d.off_vx;
+            b = -1;
+            ls = d.nx + 1;
+            ps = 1;
+        } else {
+            a = d.off_vy;
+            b = -1;
+            ls = 1;
+            ps = d.nx;
+        }
+    };
+    std::vector<std::vector<char>> covered(plan->n_domains, std::vector<char>(4, 0));
     for (int si = 0; si < plan->n_segments; ++si) {
         const pstd_plan_segment &s = plan->segments[si];
         if (s.domain < 0 || s.domain >= plan->n_domains || s.side1 >= plan->n_domains || s.side2 >= plan->n_domains ||
-            (s.direction != 0 && s.direction != 1) || (s.field != 0 && s.field != 1) || s.wlen <= 0) {
+            (s.direction != 0 && s.direction != 1) || (s.field != 0 && s.field != 1) || s.wlen <= 0 || s.wlen > 60) {
             ctx->err = "invalid segment " + std::to_string(si);
             return PSTD_ERR_INVALID;
         }
@@ -568,14 +679,10 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
             ctx->err = "segment line range outside its domain";
             return PSTD_ERR_INVALID;
         }
-        SegDev g{};
-        g.domain = s.domain;
+        TileDesc g{};
+        g.kind = 0;
         g.dir = s.direction;
-        g.field = s.field;
-        g.side1 = s.side1;
-        g.side2 = s.side2;
-        g.off1 = s.side1_offset;
-        g.off2 = s.side2_offset;
+        g.vel = s.field;
         g.wlen = s.wlen;
         g.n_main = s.field == 0 ? n : n + 1;
         g.n_out = s.field == 0 ? n + 1 : n;
@@ -584,6 +691,8 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         g.c_m1 = s.c_main1;
         g.c_s2 = s.c_side2;
         g.c_m2 = s.c_main2;
+        g.rho = dm.rho;
+        g.is_pml = dm.is_pml;
         if (g.F < 128 || g.F > 8192) {
             ctx->err = "FFT length " + std::to_string(g.F) + " outside 128..8192";
             return PSTD_ERR_UNSUPPORTED;
@@ -596,6 +705,8 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
             ctx->err = "window longer than the domain";
             return PSTD_ERR_INVALID;
         }
+        src_addr(dm, s.direction, s.field, g.m_a, g.m_b, g.m_ls, g.m_ps);
+        g.s1_len = g.s2_len = -1;
         // neighbour bounds: lines and halo columns must exist in the neighbour
         for (int side = 0; side < 2; ++side) {
             int nb = side == 0 ? s.side1 : s.side2;
@@ -608,7 +719,27 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
                 ctx->err = "segment reads outside its neighbour domain";
                 return PSTD_ERR_INVALID;
             }
+            if (side == 0) {
+                src_addr(dn, s.direction, s.field, g.s1_a, g.s1_b, g.s1_ls, g.s1_ps);
+                g.s1_line = o;
+                g.s1_len = nlen;
+            } else {
+                src_addr(dn, s.direction, s.field, g.s2_a, g.s2_b, g.s2_ls, g.s2_ps);
+                g.s2_line = o;
+                g.s2_len = nlen;
+            }
         }
+        // output field: X pressure -> vx, X velocity -> px, Y pressure -> vy, Y velocity -> py
+        if (s.direction == 0) {
+            g.o_off = s.field == 0 ? dm.off_vx : dm.off_px;
+            g.o_ls = s.field == 0 ? dm.nx + 1 : dm.nx;
+            g.o_ps = 1;
+        } else {
+            g.o_off = s.field == 0 ? dm.off_vy : dm.off_py;
+            g.o_ls = 1;
+            g.o_ps = dm.nx;
+        }
+        covered[s.domain][s.direction == 0 ? (s.field == 0 ? 2 : 0) : (s.field == 0 ? 3 : 1)] = 1;
         auto fk = std::make_tuple(g.F, s.table_len, s.field);
         auto it = filt_cache.find(fk);
         if (it == filt_cache.end()) {
@@ -634,13 +765,11 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
             ftab.insert(ftab.end(), w.begin(), w.end());
         }
         g.win_off = wt->second;
-        ctx->segs.push_back(g);
-        // tiles: 2 lines per pair, P pairs per block
+        // tiles: NL = 2 * P lines per block
         const int P = g.F >= 4096 ? 1 : kSmallThreads / (g.F / 8);
         const int per = 2 * P;
         for (int l = s.line_begin; l < s.line_end; l += per) {
-            TileDev t{};
-            t.seg = si;
+            TileDesc t = g;
             t.line0 = l;
             t.count = std::min(per, s.line_end - l);
             if (g.F == 4096) ctx->tiles_4096.push_back(t);
@@ -649,26 +778,24 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         }
     }
     // passthrough tiles for fields no segment updates (rigid domains, directions
-    // openPSTD does not update)
-    std::vector<std::vector<char>> covered(plan->n_domains, std::vector<char>(4, 0));
-    for (const SegDev &g : ctx->segs) {
-        int f = g.dir == 0 ? (g.field == 0 ? PSTD_FIELD_VX : PSTD_FIELD_PX) : (g.field == 0 ? PSTD_FIELD_VY : PSTD_FIELD_PY);
-        covered[g.domain][f - 1] = 1;
-    }
-    for (int i = 0; i < plan->n_domains; ++i)
-        for (int f = PSTD_FIELD_PX; f <= PSTD_FIELD_VY; ++f) {
-            if (covered[i][f - 1]) continue;
-            int n = field_size(plan->domains[i], f);
-            for (int e = 0; e < n; e += kPassChunk) {
-                TileDev t{};
-                t.seg = -1;
+    // openPSTD does not update).  covered[] index: 0 px, 1 py, 2 vx, 3 vy
+    for (int i = 0; i < plan->n_domains; ++i) {
+        const DomainDev &d = ctx->doms[i];
+        const int offs[4] = {d.off_px, d.off_py, d.off_vx, d.off_vy};
+        const int sizes[4] = {d.nx * d.ny, d.nx * d.ny, d.ny * (d.nx + 1), (d.ny + 1) * d.nx};
+        for (int f = 0; f < 4; ++f) {
+            if (covered[i][f]) continue;
+            for (int e = 0; e < sizes[f]; e += kPassChunk) {
+                TileDesc t{};
+                t.kind = 1;
                 t.line0 = e;
-                t.count = std::min(kPassChunk, n - e);
-                t.domain = i;
-                t.field = f;
+                t.count = std::min(kPassChunk, sizes[f] - e);
+                t.o_off = offs[f];
+                t.is_pml = d.is_pml;
                 ctx->tiles_small.push_back(t);
             }
         }
+    }
     if (ctab.empty()) ctab.push_back(make_float2(0, 0));
     if (ftab.empty()) ftab.push_back(0);
     // PML factor arena (1 where no PML)
@@ -715,20 +842,12 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     HIP_TRY(hipMemcpyAsync(ctx->ctab, ctab.data(), ctab.size() * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMalloc(&ctx->ftab, ftab.size() * sizeof(float)));
     HIP_TRY(hipMemcpyAsync(ctx->ftab, ftab.data(), ftab.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMalloc(&ctx->d_doms, std::max<size_t>(1, ctx->doms.size()) * sizeof(DomainDev)));
-    if (!ctx->doms.empty())
-        HIP_TRY(hipMemcpyAsync(ctx->d_doms, ctx->doms.data(), ctx->doms.size() * sizeof(DomainDev),
-                               hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMalloc(&ctx->d_segs, std::max<size_t>(1, ctx->segs.size()) * sizeof(SegDev)));
-    if (!ctx->segs.empty())
-        HIP_TRY(hipMemcpyAsync(ctx->d_segs, ctx->segs.data(), ctx->segs.size() * sizeof(SegDev),
-                               hipMemcpyHostToDevice, ctx->stream));
-    std::vector<TileDev> all = ctx->tiles_small;
+    std::vector<TileDesc> all = ctx->tiles_small;
     all.insert(all.end(), ctx->tiles_4096.begin(), ctx->tiles_4096.end());
     all.insert(all.end(), ctx->tiles_8192.begin(), ctx->tiles_8192.end());
-    HIP_TRY(hipMalloc(&ctx->d_tiles, std::max<size_t>(1, all.size()) * sizeof(TileDev)));
+    HIP_TRY(hipMalloc(&ctx->d_tiles, std::max<size_t>(1, all.size()) * sizeof(TileDesc)));
     if (!all.empty())
-        HIP_TRY(hipMemcpyAsync(ctx->d_tiles, all.data(), all.size() * sizeof(TileDev), hipMemcpyHostToDevice,
+        HIP_TRY(hipMemcpyAsync(ctx->d_tiles, all.data(), all.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
                                ctx->stream));
     HIP_TRY(hipMalloc(&ctx->d_cells, std::max<size_t>(1, cells.size()) * sizeof(int2)));
     if (!cells.empty())
@@ -805,17 +924,20 @@ int pstd_ctx_read_frame(pstd_ctx *ctx, int domain, float *out, size_t count) {
     return pstd_ctx_get_field(ctx, domain, PSTD_FIELD_P0, out, count);
 }
 
-static int advance_impl(pstd_ctx *ctx, int steps, float *samples) {
+static int advance_impl(pstd_ctx *ctx, int steps, float *samples, std::vector<hipEvent_t> *events = nullptr) {
     if (!ctx || !ctx->planned) return PSTD_ERR_INVALID;
     if (steps < 0) return PSTD_ERR_INVALID;
     HIP_TRY(hipSetDevice(ctx->device));
     int done = 0;
+    size_t ev = 0;
     while (done < steps) {
         const int chunk = std::min(steps - done, ctx->sample_cap);
         for (int s = 0; s < chunk; ++s) {
             for (int k = 0; k < 6; ++k) {
+                if (events) HIP_TRY(hipEventRecord((*events)[ev++], ctx->stream));
                 int rc = launch_stage(ctx, k, s == 0);
                 if (rc) return rc;
+                if (events) HIP_TRY(hipEventRecord((*events)[ev++], ctx->stream));
                 if (k == 0) ctx->p0_explicit = false;
             }
             if (ctx->nrec > 0) {
@@ -852,5 +974,35 @@ int pstd_ctx_synchronize(pstd_ctx *ctx) {
 }
 
 int pstd_ctx_steps_done(const pstd_ctx *ctx) { return ctx ? ctx->steps_done : -1; }
+
+int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int *launches) {
+    if (!ctx || !ctx->planned || steps <= 0 || !mean_stage_ms) return PSTD_ERR_INVALID;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<hipEvent_t> events((size_t)steps * 12);
+    for (auto &e : events) HIP_TRY(hipEventCreate(&e));
+    int rc = advance_impl(ctx, steps, nullptr, &events);
+    double total = 0;
+    if (rc == PSTD_OK) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i + 1 < events.size(); i += 2) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, events[i], events[i + 1]));
+            total += ms;
+        }
+    }
+    for (auto &e : events) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    *mean_stage_ms = total / (double)(events.size() / 2);
+    if (launches) *launches = (int)(events.size() / 2);
+    return PSTD_OK;
+}
+
+long long pstd_ctx_cells_per_stage(const pstd_ctx *ctx) {
+    if (!ctx || !ctx->planned) return -1;
+    long long n = 0;
+    for (const pstd_plan_domain &d : ctx->plan_domains)
+        if (!d.is_rigid) n += (long long)d.nx * d.ny;
+    return n;
+}
 
 }  // extern "C"

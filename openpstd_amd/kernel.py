@@ -209,6 +209,19 @@ class DeviceSolver:
     def steps_done(self):
         return lib().pstd_ctx_steps_done(self.ctx)
 
+    @property
+    def cells_per_stage(self):
+        """Cells of the non-rigid domains (PML included): the metric's cell count."""
+        return int(lib().pstd_ctx_cells_per_stage(self.ctx))
+
+    def profile_stages(self, steps):
+        """Advances `steps` steps timing each RK-stage launch with HIP events;
+        returns (mean stage-launch duration in ms, launches)."""
+        ms = C.c_double()
+        n = C.c_int()
+        _check(lib().pstd_ctx_profile_stages(self.ctx, steps, C.byref(ms), C.byref(n)), self._err)
+        return ms.value, n.value
+
     def field(self, i, name):
         out = np.zeros(self.kernel._shape(i, name), dtype=np.float32)
         _check(lib().pstd_ctx_get_field(self.ctx, i, T.FIELD_IDS[name], out.ctypes.data_as(C.POINTER(C.c_float)),

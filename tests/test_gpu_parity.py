@@ -1,9 +1,11 @@
 """HIP solver (libopenpstd_amd.so on the GPU) against the CPU oracle.
 
-Tolerances (FP32 device arithmetic):
-  * fields after a few steps: rel-L2 <= 2e-6 per domain field vs the FP64 oracle;
-  * receiver series: rel-L2 <= 1e-4 vs the FP64 oracle (the metric's accuracy bar),
-    and no worse than 2x the FP32 oracle's own error vs FP64 plus 1e-5.
+Tolerances (FP32 device arithmetic; the source's peak pressure is 1):
+  * fields: rel-L2 over all domains of a field vs the FP64 oracle <= 2e-6 after
+    a few steps (2e-5 after tens), max-abs per domain <= 10x that of the peak;
+  * receiver series with signal (peak > 1e-2): rel-L2 <= 1e-4 vs the FP64
+    oracle (the metric's accuracy bar), and no worse than 2x the CPU FP32
+    oracle's own error plus 1e-5; every series: max-abs <= 2e-6.
 """
 import math
 import os
@@ -28,6 +30,14 @@ def rel(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
+def assert_series(got, want, tol=1e-4):
+    assert got.shape == want.shape
+    assert np.isfinite(got).all()
+    assert np.abs(got - want).max() <= 2e-6, np.abs(got - want).max()
+    if np.abs(want).max() > 1e-2:
+        assert rel(got, want) <= tol, rel(got, want)
+
+
 def device(conf):
     k = P.PSTDKernel(GPU=True)
     k.initialize_kernel(conf)
@@ -35,14 +45,22 @@ def device(conf):
 
 
 def compare_fields(k, dev, o, tol, names=("p0", "px0", "py0", "vx0", "vy0")):
+    """Per field name: rel-L2 over all domains together <= tol, and every
+    domain's max-abs error <= 10 * tol * (largest |value| of that field in the
+    scene).  Domains the wave has not reached hold roundoff-level values, so a
+    per-domain relative error would divide by ~0."""
     worst = 0.0
-    for i in range(k.domain_count()):
-        for f in names:
-            a, b = dev.field(i, f), o.field(i, f)
-            scale = max(np.linalg.norm(b), 1e-6 * math.sqrt(b.size))
-            e = np.linalg.norm(a - b) / scale
-            worst = max(worst, e)
-            assert e <= tol, (i, f, e)
+    for f in names:
+        pairs = [(dev.field(i, f), o.field(i, f)) for i in range(k.domain_count())]
+        a = np.concatenate([x.ravel() for x, _ in pairs])
+        b = np.concatenate([y.ravel() for _, y in pairs])
+        e = rel(a, b)
+        amax = np.abs(b).max()
+        print(f"  {f}: rel-L2 {e:.2e}  max-abs {np.abs(a - b).max():.2e} of {amax:.2e}")
+        assert e <= tol, (f, e)
+        for i, (x, y) in enumerate(pairs):
+            assert np.abs(x - y).max() <= 10 * tol * amax, (i, f)
+        worst = max(worst, e)
     return worst
 
 
@@ -78,7 +96,7 @@ def test_c2_short(seed):
     o = ob.OracleScene(conf, 64, THREADS)
     want = o.step(steps)
     compare_fields(k, dev, o, 2e-5)
-    assert rel(got, want) <= 1e-4
+    assert_series(got, want)
 
 
 def test_c2_full_golden():
@@ -98,7 +116,7 @@ def test_c1_full_golden():
     conf = W.c1(500)
     k, dev = device(conf)
     got = dev.advance(500)
-    assert rel(got, g["fp64"]) <= 1e-4
+    assert_series(got, g["fp64"])
 
 
 @pytest.mark.parametrize("idx", range(4))
@@ -109,7 +127,7 @@ def test_c4_scenes(idx):
     o = ob.OracleScene(conf, 64, THREADS)
     want = o.step(40)
     compare_fields(k, dev, o, 2e-5)
-    assert rel(got, want) <= 1e-4
+    assert_series(got, want)
 
 
 def test_default_conf_partial_pmls():
@@ -170,24 +188,28 @@ def test_run_callbacks_contract():
     o = ob.OracleScene(conf, 64)
     frames = []
     series = o.step(steps, frames)
+    assert len(frames) == len(cb.frames)
     for (f1, d1, a), (f2, d2, b) in zip(cb.frames, frames):
         assert (f1, d1) == (f2, d2)
-        assert rel(a, b) < 1e-5
+        assert np.abs(a - b).max() <= 2e-6
     for r in cb.samples:
         for s, v in cb.samples[r]:
-            assert v == pytest.approx(series[s, r], rel=1e-4, abs=1e-9)
+            assert v == pytest.approx(series[s, r], rel=1e-4, abs=2e-6)
     # device state stays readable after run (last frame)
-    assert rel(k.field(0, "p0").ravel(), o.field(0, "p0").ravel()) < 1e-5
+    assert np.abs(k.field(0, "p0").ravel() - o.field(0, "p0").ravel()).max() <= 2e-6
 
 
 def test_reference_receiver_mode():
-    conf = W.c2(0, 30)
+    """openPSTD's transposed receiver read: receiver (70, 50) samples the cell
+    x = 50, y = 70, next to the source of seed 1 at (45.5, 70.5)."""
+    conf = W.c2(1, 60)
     conf.receiver_mode = T.PSTD_RECEIVER_REFERENCE
-    conf.Receivers = [(W._world_pos(40.25), W._world_pos(30.25), 0.0), (W._world_pos(100.25), W._world_pos(20.25), 0.0)]
+    conf.Receivers = [(W._world_pos(70.25), W._world_pos(50.25), 0.0), (W._world_pos(60.25), W._world_pos(30.25), 0.0)]
     k, dev = device(conf)
-    got = dev.advance(30)
-    want = ob.OracleScene(conf, 64).step(30)
-    assert rel(got, want) <= 1e-4
+    got = dev.advance(60)
+    want = ob.OracleScene(conf, 64, THREADS).step(60)
+    assert np.abs(want).max() > 0.05
+    assert_series(got, want)
 
 
 def test_set_field_midrun_p0():
