@@ -73,7 +73,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the PSTD loop here runs on a single MI355X; "
-              "multi-GPU sharding is a design note only (see DESIGN.md).", file=sys.stderr)
+              "sharding across GPUs is not built (one process, one device).", file=sys.stderr)
         sys.exit(2)
 
     import openpstd_amd as P

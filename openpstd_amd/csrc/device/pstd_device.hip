@@ -75,7 +75,7 @@ struct TileDesc {
 };
 
 struct StageArgs {
-    const TileDesc *tiles;
+    const TileDesc *__restrict__ tiles;
     const float2 *ctab;
     const float *ftab;
     const float *p0a;  // p0 = p0a[a + idx] + (p0b ? p0b[b + idx] : 0)
@@ -87,6 +87,12 @@ struct StageArgs {
     float s;           // dt * alpha_k
     float c2;          // c1 * c1
     int last;          // last RK stage: also write tn = out * pml
+    // receiver sampling of the previous step (first stage of a step): p0 = px + py
+    // of the previous step's raw stage-5 output (kernel/Solver.cpp:86-91)
+    const int2 *cells;
+    int nrec;
+    float *samples;    // samples[slot * nrec + r], slot < 0: no sampling in this launch
+    int slot;
 };
 
 // ------------------------------------------------------------- device code
@@ -119,8 +125,15 @@ __device__ __forceinline__ int stage_at(const TileGeom &g, int ll, int pos) {
     return g.layout == 0 ? (ll << (g.logT + 3)) + pos : (pos << (g.logP + 1)) + ll;
 }
 
-__device__ __forceinline__ void gather_lines(const StageArgs &a, const TileDesc &td, const TileGeom &g, int p, int t,
-                                             float *stage, float2 (&v)[8]) {
+struct GatherRegs {
+    float ra[8][2], rb[8][2], win[8];
+    bool okv[8][2];
+    bool hasB;
+};
+
+// Issues the gather loads of thread (p, t) (no waits).
+__device__ __forceinline__ void gather_issue(const StageArgs &a, const TileDesc &td, const TileGeom &g, int p, int t,
+                                             GatherRegs &G) {
     const int w = td.wlen, nm = td.n_main, vel = td.vel, count = td.count;
     const int m_a = td.m_a, m_b = td.m_b, m_ls = td.m_ls, m_ps = td.m_ps;
     const int s1_a = td.s1_a, s1_b = td.s1_b, s1_ls = td.s1_ls, s1_ps = td.s1_ps, s1_line = td.s1_line;
@@ -130,10 +143,9 @@ __device__ __forceinline__ void gather_lines(const StageArgs &a, const TileDesc 
     const float *A = vel ? a.vsrc : a.p0a;
     const bool hasB = !vel && a.p0b != nullptr;
     const float *B = hasB ? a.p0b : A;
+    G.hasB = hasB;
     const int line_a = td.line0 + 2 * p;
     const bool ok0 = 2 * p < count, ok1 = 2 * p + 1 < count;
-    float ra[8][2], rb[8][2], win[8];
-    bool okv[8][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         const int pos = t + (m << g.logT);
@@ -149,21 +161,29 @@ __device__ __forceinline__ void gather_lines(const StageArgs &a, const TileDesc 
         const bool v0 = vr && ok0, v1 = vr && ok1;
         const int x0 = v0 ? ba + i0 : m_a, x1 = v1 ? ba + i0 + ls : m_a;
         const int y0 = v0 && hasB ? bb + i0 : m_a, y1 = v1 && hasB ? bb + i0 + ls : m_a;
-        ra[m][0] = A[x0];
-        ra[m][1] = A[x1];
-        rb[m][0] = B[y0];
-        rb[m][1] = B[y1];
-        okv[m][0] = v0;
-        okv[m][1] = v1;
-        win[m] = a.ftab[win_off + (left ? pos : ((right && pos < 2 * w + nm) ? pos - nm + 1 : 0))];
+        G.ra[m][0] = A[x0];
+        G.ra[m][1] = A[x1];
+        G.rb[m][0] = B[y0];
+        G.rb[m][1] = B[y1];
+        G.okv[m][0] = v0;
+        G.okv[m][1] = v1;
+        G.win[m] = a.ftab[win_off + (left ? pos : ((right && pos < 2 * w + nm) ? pos - nm + 1 : 0))];
     }
+}
+
+// Stages the raw values, then builds the windowed complex pairs.
+__device__ __forceinline__ void gather_finish(const TileDesc &td, const TileGeom &g, int p, int t, const GatherRegs &G,
+                                              float *stage, float2 (&v)[8]) {
+    const int w = td.wlen, nm = td.n_main, vel = td.vel;
+    const float c_s1 = td.c_s1, c_m1 = td.c_m1, c_s2 = td.c_s2, c_m2 = td.c_m2;
+    const bool hasB = G.hasB;
     float raw[8][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         const int pos = t + (m << g.logT);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            raw[m][q] = okv[m][q] ? (hasB ? ra[m][q] + rb[m][q] : ra[m][q]) : 0.f;
+            raw[m][q] = G.okv[m][q] ? (hasB ? G.ra[m][q] + G.rb[m][q] : G.ra[m][q]) : 0.f;
             stage[stage_at(g, 2 * p + q, pos)] = raw[m][q];
         }
     }
@@ -180,7 +200,7 @@ __device__ __forceinline__ void gather_lines(const StageArgs &a, const TileDesc 
         for (int q = 0; q < 2; ++q) {
             const float refl = stage[stage_at(g, 2 * p + q, w + mpos)];
             const float r = raw[m][q];
-            out[q] = mid ? r : ((left || right) ? (r * cs + refl * cm) * win[m] : 0.f);
+            out[q] = mid ? r : ((left || right) ? (r * cs + refl * cm) * G.win[m] : 0.f);
         }
         v[m] = make_float2(out[0], out[1]);
     }
@@ -207,43 +227,51 @@ __device__ __forceinline__ void prefetch_twiddles(float2 *twr, int t, const floa
     fft_load_twiddles<F>(twr, t, tw);
 }
 
-// Solver::update_field_values (kernel/Solver.cpp:174-181): branch-free loads of
-// the t_n values (and the PML factors in the last stage), then the stores.
-__device__ __forceinline__ void write_updates(const StageArgs &a, const TileDesc &td, const TileGeom &g,
-                                              const float2 (&v)[8], int p, int t) {
+// Solver::update_field_values (kernel/Solver.cpp:174-181).  The t_n values
+// (and, in the last stage, the PML factors) depend only on the descriptor, so
+// they are loaded at the start of the tile and stay in registers.
+struct UpdateRegs {
+    int idx[8][2];
+    float tnv[8][2], pm[8][2];
+};
+
+__device__ __forceinline__ void prefetch_updates(const StageArgs &a, const TileDesc &td, const TileGeom &g, int p,
+                                                 int t, UpdateRegs &u) {
     const int w = td.wlen, n_out = td.n_out, o_off = td.o_off, o_ls = td.o_ls, o_ps = td.o_ps;
     const bool ok0 = 2 * p < td.count, ok1 = 2 * p + 1 < td.count;
     const int base = o_off + (td.line0 + 2 * p) * o_ls;
-    const float rho = td.rho, c2 = a.c2, s = a.s;
-    const bool vel = td.vel != 0;
     const bool use_pml = a.last && td.is_pml;
-    int idx[8][2];
-    float tnv[8][2], pm[8][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         const int pos = t + (m << g.logT) - w;
         const bool okp = pos >= 0 && pos < n_out;
         const int i0 = base + pos * o_ps;
-        idx[m][0] = okp && ok0 ? i0 : -1;
-        idx[m][1] = okp && ok1 ? i0 + o_ls : -1;
+        u.idx[m][0] = okp && ok0 ? i0 : -1;
+        u.idx[m][1] = okp && ok1 ? i0 + o_ls : -1;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const int id = idx[m][q] >= 0 ? idx[m][q] : o_off;
-            tnv[m][q] = a.tn[id];
-            pm[m][q] = use_pml ? a.pml[id] : 1.f;
+            const int id = u.idx[m][q] >= 0 ? u.idx[m][q] : o_off;
+            u.tnv[m][q] = a.tn[id];
+            u.pm[m][q] = use_pml ? a.pml[id] : 1.f;
         }
     }
+}
+
+__device__ __forceinline__ void finish_updates(const StageArgs &a, const TileDesc &td, const float2 (&v)[8],
+                                               const UpdateRegs &u) {
+    const float rho = td.rho, c2 = a.c2, s = a.s;
+    const bool vel = td.vel != 0;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const int id = idx[m][q];
+            const int id = u.idx[m][q];
             const float L = q ? v[m].y : v[m].x;
             const float upd = vel ? L * rho * c2 : L / rho;
-            const float nv = tnv[m][q] - s * upd;
+            const float nv = u.tnv[m][q] - s * upd;
             if (id >= 0) {
                 a.out[id] = nv;
-                if (a.last) a.tn[id] = nv * pm[m][q];
+                if (a.last) a.tn[id] = nv * u.pm[m][q];
             }
         }
     }
@@ -256,7 +284,9 @@ __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDe
     const int tid = threadIdx.x;
     const int p = g.layout == 0 ? tid >> g.logT : tid & (g.P - 1);
     const int t = g.layout == 0 ? tid & (g.T - 1) : tid >> g.logP;
-    // data-independent loads first: filter and twiddles
+    // loads in order of need: line data, filter and twiddles, update operands
+    GatherRegs G;
+    gather_issue(a, td, g, p, t, G);
     float2 h[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) h[m] = a.ctab[td.h_off + t + m * g.T];
@@ -272,9 +302,11 @@ __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDe
         case 8192: if (FMIN <= 8192 && FMAX >= 8192) prefetch_twiddles<8192>(twr, t, tw); break;
         default: break;
     }
+    UpdateRegs u;
+    prefetch_updates(a, td, g, p, t, u);
     float *stage = reinterpret_cast<float *>(buf1);
     float2 v[8];
-    gather_lines(a, td, g, p, t, stage, v);
+    gather_finish(td, g, p, t, G, stage, v);
 #define PSTD_FFT_CASE(FV)                                                                           \
     case FV:                                                                                        \
         if constexpr (FMIN <= FV && FMAX >= FV) {                                                   \
@@ -293,7 +325,17 @@ __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDe
         default: break;
     }
 #undef PSTD_FFT_CASE
-    write_updates(a, td, g, v, p, t);
+    finish_updates(a, td, v, u);
+}
+
+// One extra workgroup of a launch samples the receivers (p0 = px0 + py0 of the
+// previous step, read from the arena `p0a`/`p0b` point at in the first stage).
+__device__ void sample_receivers(const StageArgs &a) {
+    if (a.slot < 0) return;
+    for (int r = threadIdx.x; r < a.nrec; r += blockDim.x) {
+        const int2 c = a.cells[r];
+        a.samples[(size_t)a.slot * a.nrec + r] = a.p0a[c.x] + a.p0b[c.y];
+    }
 }
 
 // Fields no segment updates keep their t_n value (openPSTD leaves their L at 0,
@@ -325,7 +367,11 @@ constexpr int lds_small_floats2() { return 2 * 8 * kSmallThreads; }
 
 __global__ void __launch_bounds__(kSmallThreads) stage_small_kernel(StageArgs a) {
     __shared__ float2 lds[lds_small_floats2()];
-    const TileDesc td = a.tiles[blockIdx.x];
+    const TileDesc &td = a.tiles[blockIdx.x];
+    if (td.kind == 2) {
+        sample_receivers(a);
+        return;
+    }
     if (td.kind == 1) {
         passthrough_tile(a, td);
         return;
@@ -336,7 +382,7 @@ __global__ void __launch_bounds__(kSmallThreads) stage_small_kernel(StageArgs a)
 template <int F>
 __global__ void __launch_bounds__(F / 8) stage_big_kernel(StageArgs a) {
     extern __shared__ float2 lds_dyn[];
-    const TileDesc td = a.tiles[blockIdx.x];
+    const TileDesc &td = a.tiles[blockIdx.x];
     derivative_tile<F / 8, F, F>(a, td, lds_dyn, lds_dyn + F);
 }
 
@@ -475,7 +521,9 @@ void build_window(int w, int patch_error, std::vector<float> &out) {
     }
 }
 
-int launch_stage(pstd_ctx *ctx, int k, bool first_of_step) {
+// One RK stage.  sample_slot >= 0 (first stage only): the launch also records
+// the receivers of the previous step into that sample slot.
+int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
     StageArgs a{};
     a.ctab = ctx->ctab;
     a.ftab = ctx->ftab;
@@ -494,7 +542,10 @@ int launch_stage(pstd_ctx *ctx, int k, bool first_of_step) {
         a.p0b = ctx->S(in);
         a.vsrc = ctx->S(in);
     }
-    (void)first_of_step;
+    a.cells = ctx->d_cells;
+    a.nrec = ctx->nrec;
+    a.samples = ctx->d_samples;
+    a.slot = (k == 0 && !ctx->p0_explicit && ctx->nrec > 0) ? sample_slot : -1;
     a.tn = ctx->TN();
     a.out = ctx->S(outi);
     a.pml = ctx->pml;
@@ -796,6 +847,11 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
             }
         }
     }
+    if (plan->n_receivers > 0) {
+        TileDesc t{};
+        t.kind = 2;  // receiver sampling of the previous step (first stage only)
+        ctx->tiles_small.push_back(t);
+    }
     if (ctab.empty()) ctab.push_back(make_float2(0, 0));
     if (ftab.empty()) ftab.push_back(0);
     // PML factor arena (1 where no PML)
@@ -935,17 +991,18 @@ static int advance_impl(pstd_ctx *ctx, int steps, float *samples, std::vector<hi
         for (int s = 0; s < chunk; ++s) {
             for (int k = 0; k < 6; ++k) {
                 if (events) HIP_TRY(hipEventRecord((*events)[ev++], ctx->stream));
-                int rc = launch_stage(ctx, k, s == 0);
+                // the first stage of step s samples step s-1 of this chunk
+                int rc = launch_stage(ctx, k, s - 1);
                 if (rc) return rc;
                 if (events) HIP_TRY(hipEventRecord((*events)[ev++], ctx->stream));
                 if (k == 0) ctx->p0_explicit = false;
             }
-            if (ctx->nrec > 0) {
-                hipLaunchKernelGGL(receiver_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->S(1), ctx->d_cells,
-                                   ctx->nrec, ctx->d_samples, s);
-                HIP_TRY(hipGetLastError());
-            }
             ctx->steps_done++;
+        }
+        if (ctx->nrec > 0) {  // the chunk's last step
+            hipLaunchKernelGGL(receiver_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->S(1), ctx->d_cells, ctx->nrec,
+                               ctx->d_samples, chunk - 1);
+            HIP_TRY(hipGetLastError());
         }
         if (samples && ctx->nrec > 0) {
             HIP_TRY(hipMemcpyAsync(samples + (size_t)done * ctx->nrec, ctx->d_samples,
