@@ -174,10 +174,11 @@ int pstd_ctx_advance_async(pstd_ctx *ctx, int steps);
 int pstd_ctx_synchronize(pstd_ctx *ctx);
 /* Current p0 of one domain, row-major (Solver::get_pressure_vector). */
 int pstd_ctx_read_frame(pstd_ctx *ctx, int domain, float *out, size_t count);
-/* Advances `steps` time steps like pstd_ctx_advance_async, with a HIP event
- * pair on the solver stream around every RK-stage kernel launch; returns the
- * mean GPU duration of one stage launch (ms) and the number of launches timed.
- * For measurement; the state advances exactly as with pstd_ctx_advance. */
+/* Advances `steps` time steps like pstd_ctx_advance, with one HIP event pair
+ * on the solver stream around the whole run, and waits for it; returns the
+ * GPU time of the run divided by its 6*steps RK-stage launches (ms) and that
+ * launch count.  For measurement; the state advances exactly as with
+ * pstd_ctx_advance (samples stay on the device). */
 int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int *launches);
 /* Cells updated per RK stage: sum of nx*ny over the non-rigid domains. */
 long long pstd_ctx_cells_per_stage(const pstd_ctx *ctx);

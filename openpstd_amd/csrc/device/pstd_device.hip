@@ -980,21 +980,18 @@ int pstd_ctx_read_frame(pstd_ctx *ctx, int domain, float *out, size_t count) {
     return pstd_ctx_get_field(ctx, domain, PSTD_FIELD_P0, out, count);
 }
 
-static int advance_impl(pstd_ctx *ctx, int steps, float *samples, std::vector<hipEvent_t> *events = nullptr) {
+static int advance_impl(pstd_ctx *ctx, int steps, float *samples) {
     if (!ctx || !ctx->planned) return PSTD_ERR_INVALID;
     if (steps < 0) return PSTD_ERR_INVALID;
     HIP_TRY(hipSetDevice(ctx->device));
     int done = 0;
-    size_t ev = 0;
     while (done < steps) {
         const int chunk = std::min(steps - done, ctx->sample_cap);
         for (int s = 0; s < chunk; ++s) {
             for (int k = 0; k < 6; ++k) {
-                if (events) HIP_TRY(hipEventRecord((*events)[ev++], ctx->stream));
                 // the first stage of step s samples step s-1 of this chunk
                 int rc = launch_stage(ctx, k, s - 1);
                 if (rc) return rc;
-                if (events) HIP_TRY(hipEventRecord((*events)[ev++], ctx->stream));
                 if (k == 0) ctx->p0_explicit = false;
             }
             ctx->steps_done++;
@@ -1035,22 +1032,21 @@ int pstd_ctx_steps_done(const pstd_ctx *ctx) { return ctx ? ctx->steps_done : -1
 int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int *launches) {
     if (!ctx || !ctx->planned || steps <= 0 || !mean_stage_ms) return PSTD_ERR_INVALID;
     HIP_TRY(hipSetDevice(ctx->device));
-    std::vector<hipEvent_t> events((size_t)steps * 12);
-    for (auto &e : events) HIP_TRY(hipEventCreate(&e));
-    int rc = advance_impl(ctx, steps, nullptr, &events);
-    double total = 0;
-    if (rc == PSTD_OK) {
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        for (size_t i = 0; i + 1 < events.size(); i += 2) {
-            float ms = 0;
-            HIP_TRY(hipEventElapsedTime(&ms, events[i], events[i + 1]));
-            total += ms;
-        }
-    }
-    for (auto &e : events) (void)hipEventDestroy(e);
+    hipEvent_t ev[2];
+    HIP_TRY(hipEventCreate(&ev[0]));
+    HIP_TRY(hipEventCreate(&ev[1]));
+    int rc = PSTD_OK;
+    float ms = 0;
+    if (hipEventRecord(ev[0], ctx->stream) != hipSuccess) rc = PSTD_ERR_DEVICE;
+    if (rc == PSTD_OK) rc = advance_impl(ctx, steps, nullptr);
+    if (rc == PSTD_OK && hipEventRecord(ev[1], ctx->stream) != hipSuccess) rc = PSTD_ERR_DEVICE;
+    if (rc == PSTD_OK && hipEventSynchronize(ev[1]) != hipSuccess) rc = PSTD_ERR_DEVICE;
+    if (rc == PSTD_OK && hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) rc = PSTD_ERR_DEVICE;
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
     if (rc) return rc;
-    *mean_stage_ms = total / (double)(events.size() / 2);
-    if (launches) *launches = (int)(events.size() / 2);
+    *mean_stage_ms = (double)ms / (6.0 * steps);
+    if (launches) *launches = 6 * steps;
     return PSTD_OK;
 }
 

@@ -215,8 +215,8 @@ class DeviceSolver:
         return int(lib().pstd_ctx_cells_per_stage(self.ctx))
 
     def profile_stages(self, steps):
-        """Advances `steps` steps timing each RK-stage launch with HIP events;
-        returns (mean stage-launch duration in ms, launches)."""
+        """Advances `steps` steps between two HIP events on the solver stream and
+        waits; returns (GPU time per RK-stage launch in ms, launches)."""
         ms = C.c_double()
         n = C.c_int()
         _check(lib().pstd_ctx_profile_stages(self.ctx, steps, C.byref(ms), C.byref(n)), self._err)

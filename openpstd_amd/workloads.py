@@ -67,7 +67,7 @@ def domain(x, y, nx, ny, absorption=1.0):
     return d
 
 
-def c1(steps=500):
+def room(steps=500):
     """Single 100x50-cell air domain, PML on all sides, centre source, receiver at (+5 m, +2 m)."""
     conf = PSTDConfiguration()
     base_settings(conf, steps)
@@ -77,7 +77,7 @@ def c1(steps=500):
     return conf
 
 
-def c2(seed=0, steps=2000, absorptions=(0.0, 0.5, 1.0)):
+def lshape(seed=0, steps=2000, absorptions=(0.0, 0.5, 1.0)):
     """L-shape of three 256x128 domains A(0,0) B(256,0) C(0,128); the 8 outer edges
     draw their absorption from `absorptions`; one source in A, four receivers."""
     rng = np.random.RandomState(seed)
@@ -101,7 +101,7 @@ def c2(seed=0, steps=2000, absorptions=(0.0, 0.5, 1.0)):
     return conf
 
 
-def c3(seed=0, steps=1000, n=4096, n_receivers=16):
+def square(seed=0, steps=1000, n=4096, n_receivers=16):
     """One n x n domain, PML on all sides, centre source, seeded receivers."""
     rng = np.random.RandomState(seed)
     conf = PSTDConfiguration()
@@ -121,7 +121,7 @@ def _side(rng, lo, hi):
             return v
 
 
-def c4_scene(rng, steps, lo=64, hi=1024):
+def batch_scene(rng, steps, lo=64, hi=1024):
     """One scene of 1-4 conforming domains (row, column or 2x2 grid), edge absorption U[0,1]."""
     layout = rng.randint(4)  # 0 single, 1 row, 2 column, 3 grid
     conf = PSTDConfiguration()
@@ -150,13 +150,13 @@ def c4_scene(rng, steps, lo=64, hi=1024):
     return conf
 
 
-def c4(seed=0, steps=2000, n_scenes=64, lo=64, hi=1024):
+def scene_batch(seed=0, steps=2000, n_scenes=64, lo=64, hi=1024):
     """64 independent scenes of seeded uneven rectangular domains."""
     rng = np.random.RandomState(seed)
-    return [c4_scene(rng, steps, lo, hi) for _ in range(n_scenes)]
+    return [batch_scene(rng, steps, lo, hi) for _ in range(n_scenes)]
 
 
-def c5(seed=0, steps=3000, n_sources=256, receivers_per_source=8, n=1024):
+def source_batch(seed=0, steps=3000, n_sources=256, receivers_per_source=8, n=1024):
     """One n x n geometry as two (n/2) x n domains side by side, PML everywhere;
     returns a list of configurations, one per seeded source position, each with
     its own seeded receivers."""

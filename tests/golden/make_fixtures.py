@@ -5,7 +5,7 @@
      window_w32_pe70.json      test/Kernel/kernel_functions.cpp:147
      kat_reference.json        next_2_power, get_rho_array, wavenumbers, Gaussian
 2. Long-run oracle series (FP64 CPU oracle), too slow to recompute in a test:
-     c1_receivers.npz, c2_receivers.npz
+     room_receivers.npz, lshape_receivers.npz
 
     python tests/golden/make_fixtures.py [--kat] [--series]
 """
@@ -55,7 +55,7 @@ def kat():
 def series():
     from oracle import binding as ob
     from openpstd_amd import workloads as W
-    for name, conf, steps in (("c1", W.c1(500), 500), ("c2", W.c2(0, 2000), 2000)):
+    for name, conf, steps in (("room", W.room(500), 500), ("lshape", W.lshape(0, 2000), 2000)):
         out = {}
         for prec in (64, 32):
             o = ob.OracleScene(conf, prec, threads=os.cpu_count() or 1)

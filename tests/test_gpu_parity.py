@@ -65,8 +65,8 @@ def compare_fields(k, dev, o, tol, names=("p0", "px0", "py0", "vx0", "vy0")):
 
 
 @pytest.mark.parametrize("steps", [1, 3])
-def test_c1_fields_after_steps(steps):
-    conf = W.c1(steps)
+def test_room_fields_after_steps(steps):
+    conf = W.room(steps)
     k, dev = device(conf)
     dev.advance(steps)
     o = ob.OracleScene(conf, 64)
@@ -74,23 +74,23 @@ def test_c1_fields_after_steps(steps):
     compare_fields(k, dev, o, 2e-6)
 
 
-def test_c1_receiver_series():
+def test_room_receiver_series():
     steps = 300
-    conf = W.c1(steps)
+    conf = W.room(steps)
     k, dev = device(conf)
     got = dev.advance(steps)
     o64 = ob.OracleScene(conf, 64, THREADS).step(steps)
     o32 = ob.OracleScene(conf, 32, THREADS).step(steps)
     e_dev, e_cpu32 = rel(got, o64), rel(o32, o64)
-    print(f"C1 {steps} steps: device vs FP64 {e_dev:.2e}, CPU FP32 vs FP64 {e_cpu32:.2e}")
+    print(f"room {steps} steps: device vs FP64 {e_dev:.2e}, CPU FP32 vs FP64 {e_cpu32:.2e}")
     assert e_dev <= 1e-4
     assert e_dev <= 2 * e_cpu32 + 1e-5
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
-def test_c2_short(seed):
+def test_lshape_short(seed):
     steps = 60
-    conf = W.c2(seed, steps)
+    conf = W.lshape(seed, steps)
     k, dev = device(conf)
     got = dev.advance(steps)
     o = ob.OracleScene(conf, 64, THREADS)
@@ -99,29 +99,29 @@ def test_c2_short(seed):
     assert_series(got, want)
 
 
-def test_c2_full_golden():
+def test_lshape_full_golden():
     """The bench workload at full length against the committed FP64 oracle series."""
-    g = np.load(os.path.join(HERE, "golden", "c2_receivers.npz"))
-    conf = W.c2(0, 2000)
+    g = np.load(os.path.join(HERE, "golden", "lshape_receivers.npz"))
+    conf = W.lshape(0, 2000)
     k, dev = device(conf)
     got = dev.advance(2000)
     e_dev, e_cpu32 = rel(got, g["fp64"]), rel(g["fp32"], g["fp64"])
-    print(f"C2 2000 steps: device vs FP64 {e_dev:.2e}, CPU FP32 vs FP64 {e_cpu32:.2e}")
+    print(f"L-shape 2000 steps: device vs FP64 {e_dev:.2e}, CPU FP32 vs FP64 {e_cpu32:.2e}")
     assert e_dev <= 1e-4
     assert np.isfinite(got).all()
 
 
-def test_c1_full_golden():
-    g = np.load(os.path.join(HERE, "golden", "c1_receivers.npz"))
-    conf = W.c1(500)
+def test_room_full_golden():
+    g = np.load(os.path.join(HERE, "golden", "room_receivers.npz"))
+    conf = W.room(500)
     k, dev = device(conf)
     got = dev.advance(500)
     assert_series(got, g["fp64"])
 
 
 @pytest.mark.parametrize("idx", range(4))
-def test_c4_scenes(idx):
-    conf = W.c4(7, 40, n_scenes=4, hi=300)[idx]
+def test_batch_scenes(idx):
+    conf = W.scene_batch(7, 40, n_scenes=4, hi=300)[idx]
     k, dev = device(conf)
     got = dev.advance(40)
     o = ob.OracleScene(conf, 64, THREADS)
@@ -171,7 +171,7 @@ def test_long_lines(n):
 def test_run_callbacks_contract():
     """PSTDKernel.run: the callback sequence of SingleThreadSolver (kernel/Solver.cpp:62-93)."""
     steps = 12
-    conf = W.c2(1, steps)
+    conf = W.lshape(1, steps)
     conf.Settings.SetSaveNth(5)
     k = P.PSTDKernel(GPU=True)
     k.initialize_kernel(conf)
@@ -202,7 +202,7 @@ def test_run_callbacks_contract():
 def test_reference_receiver_mode():
     """openPSTD's transposed receiver read: receiver (70, 50) samples the cell
     x = 50, y = 70, next to the source of seed 1 at (45.5, 70.5)."""
-    conf = W.c2(1, 60)
+    conf = W.lshape(1, 60)
     conf.receiver_mode = T.PSTD_RECEIVER_REFERENCE
     conf.Receivers = [(W._world_pos(70.25), W._world_pos(50.25), 0.0), (W._world_pos(60.25), W._world_pos(30.25), 0.0)]
     k, dev = device(conf)
@@ -214,7 +214,7 @@ def test_reference_receiver_mode():
 
 def test_set_field_midrun_p0():
     """Host writes between steps keep openPSTD semantics (p0 is its own array)."""
-    conf = W.c1(10)
+    conf = W.room(10)
     k, dev = device(conf)
     o = ob.OracleScene(conf, 64)
     dev.advance(3)

@@ -13,11 +13,11 @@ from test_oracle_kat import _domain_test_conf
 
 
 def _scenes():
-    yield "c1", W.c1(20)
+    yield "room", W.room(20)
     for seed in range(4):
-        yield f"c2_seed{seed}", W.c2(seed, 20)
-    for i, c in enumerate(W.c4(1, 20, n_scenes=8, hi=260)):
-        yield f"c4_{i}", c
+        yield f"lshape_seed{seed}", W.lshape(seed, 20)
+    for i, c in enumerate(W.scene_batch(1, 20, n_scenes=8, hi=260)):
+        yield f"batch_{i}", c
     yield "default_conf", P.PSTDConfiguration.CreateDefaultConf()
     yield "domain_test", _domain_test_conf()
 
@@ -89,7 +89,7 @@ def test_segments_cover_every_line_once(name, conf):
 
 def test_neighbour_stencils_match_rho_arrays():
     """Segment coefficients are openPSTD's get_rho_array entries for the pair."""
-    conf = W.c2(2, 10)
+    conf = W.lshape(2, 10)
     k = P.PSTDKernel(GPU=True)
     k.initialize_kernel(conf)
     infos = [k.domain_info(i) for i in range(k.domain_count())]
@@ -105,7 +105,7 @@ def test_neighbour_stencils_match_rho_arrays():
 
 
 def test_receiver_modes():
-    conf = W.c2(0, 10)
+    conf = W.lshape(0, 10)
     for mode in (T.PSTD_RECEIVER_PHYSICAL, T.PSTD_RECEIVER_REFERENCE):
         conf.receiver_mode = mode
         conf.Receivers = [(W._world_pos(40.25), W._world_pos(30.25), 0.0)]
@@ -124,7 +124,7 @@ def test_receiver_modes():
 
 
 def test_receiver_outside_every_domain():
-    conf = W.c1(5)
+    conf = W.room(5)
     conf.Receivers = [(500.0, 500.0, 0.0)]
     with pytest.raises(P.PSTDError, match="outside"):
         P.PSTDKernel().initialize_kernel(conf)
@@ -149,7 +149,7 @@ def test_not_configured():
 
 def test_cpu_solver_not_provided():
     k = P.PSTDKernel(GPU=False)
-    k.initialize_kernel(W.c1(5))
+    k.initialize_kernel(W.room(5))
     with pytest.raises(P.PSTDError) as e:
         k.run(P.RecordingCallback())
     assert e.value.status == T.PSTD_ERR_UNSUPPORTED
@@ -160,5 +160,5 @@ def test_settings_mirror():
     assert s.GetWindowSize() == 32
     assert s.GetTimeStep() == pytest.approx(0.5 * 0.2 / 340, rel=1e-6)
     assert s.GetBandWidth() == pytest.approx(8.67, rel=1e-6)
-    c1 = W.c1()
-    assert c1.num_steps() == 500 and W.c2().num_steps() == 2000
+    room = W.room()
+    assert room.num_steps() == 500 and W.lshape().num_steps() == 2000
