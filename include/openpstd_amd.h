@@ -160,6 +160,14 @@ int pstd_ctx_create(int device, pstd_ctx **out);
 /* Uploads the scene plan: allocates device state (zeroed), builds the device
  * plan cache (derivative filters, windows, twiddles, line tiles). */
 int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan);
+/* Uploads several independent scenes as one plan: every RK stage of all of
+ * them is one launch.  Domains and receivers are numbered across the scenes in
+ * order (scene 0's domains first); segments name domains of their own scene.
+ * The scenes must share dx, dt, c1, patch error and RK coefficients.  No
+ * openPSTD counterpart: openPSTD's Solver advances one Scene
+ * (kernel/Solver.cpp:62-93); a batch advances each scene exactly as that loop
+ * does. */
+int pstd_ctx_upload_plans(pstd_ctx *ctx, const pstd_plan_desc *plans, int n_plans);
 /* Host <-> device copies of one field of one domain (FieldValues::current). */
 int pstd_ctx_set_field(pstd_ctx *ctx, int domain, int field, const float *host, size_t count);
 int pstd_ctx_get_field(pstd_ctx *ctx, int domain, int field, float *host, size_t count);
@@ -180,6 +188,25 @@ int pstd_ctx_read_frame(pstd_ctx *ctx, int domain, float *out, size_t count);
  * launch count.  For measurement; the state advances exactly as with
  * pstd_ctx_advance (samples stay on the device). */
 int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int *launches);
+/* Replays whole blocks of `steps` time steps as one captured hipGraph (0 = the
+ * stage launches are issued one by one, the default).  The graph is captured
+ * on first use after the first time step and dropped on re-upload.  Results
+ * are identical either way. */
+int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps);
+/* Launch shape of the stage kernel.  pstd_ctx_upload_plan(s) times the
+ * candidates on the zeroed state and keeps the fastest (the autotuner); these
+ * calls override or report its choice.  Results are identical for every
+ * choice.
+ *  - tile order: 1 gives every XCD a contiguous run of tiles (neighbouring
+ *    lines share its L2), 0 the plain round-robin order;
+ *  - kernel flavour: 0 loads every descriptor-dependent operand (filter,
+ *    twiddles, t_n) before the transforms (short tiles, latency-bound scenes),
+ *    1 loads them where they are used (fewer registers, more resident waves,
+ *    throughput-bound scenes).
+ * pstd_ctx_get_tuning reports the choice and its timed GPU ms per stage. */
+int pstd_ctx_set_tile_order(pstd_ctx *ctx, int xcd_contiguous);
+int pstd_ctx_set_kernel_flavour(pstd_ctx *ctx, int lean);
+int pstd_ctx_get_tuning(const pstd_ctx *ctx, int32_t *lean, int32_t *xcd_contiguous, double *stage_ms);
 /* Cells updated per RK stage: sum of nx*ny over the non-rigid domains. */
 long long pstd_ctx_cells_per_stage(const pstd_ctx *ctx);
 /* Number of time steps advanced since upload. */
@@ -227,6 +254,11 @@ int pstd_kernel_read_pml(pstd_kernel *k, int index, int which, float *out, size_
  * by the kernel; valid until the next prepare/run/destroy).  For callers that
  * drive the time loop themselves (benchmarks, stepwise inspection). */
 int pstd_kernel_prepare(pstd_kernel *k, pstd_ctx **ctx);
+/* Builds one device context that advances the scenes of `n` configured
+ * kernels together (pstd_ctx_upload_plans), initial fields set, frame 0 next.
+ * The caller owns *ctx (pstd_ctx_destroy).  Domain d of scene i is domain
+ * (sum of the earlier scenes' domain counts) + d; receivers likewise. */
+int pstd_batch_prepare(pstd_kernel *const *kernels, int n, int device, pstd_ctx **ctx);
 /* Plan the kernel hands to the device (segments etc.) — for tests. */
 int pstd_kernel_segment_count(pstd_kernel *k);
 int pstd_kernel_get_segment(pstd_kernel *k, int i, pstd_plan_segment *out);

@@ -7,9 +7,9 @@ from .config import (PSTDSettings, DomainConfEdge, DomainConf, PSTDConfiguration
                      PSTD_DOMAIN_SIDE_TOP, PSTD_DOMAIN_SIDE_BOTTOM, PSTD_DOMAIN_SIDE_LEFT,
                      PSTD_DOMAIN_SIDE_RIGHT, PSTD_DOMAIN_SIDE_ALL, PSTD_DOMAIN_SIDE_NONE)
 from .kernel import (PSTDKernel, KernelInterface, KernelCallback, KernelCallbackLog, SimulationMetadata,
-                     PSTDKernelNotConfiguredException, DeviceSolver, RecordingCallback)
+                     PSTDKernelNotConfiguredException, DeviceSolver, BatchSolver, RecordingCallback)
 from ._lib import PSTDError
 
 __all__ = ["PSTDSettings", "DomainConfEdge", "DomainConf", "PSTDConfiguration", "RK_COEFFICIENTS", "PSTDKernel",
            "KernelInterface", "KernelCallback", "KernelCallbackLog", "SimulationMetadata",
-           "PSTDKernelNotConfiguredException", "DeviceSolver", "RecordingCallback", "PSTDError"]
+           "PSTDKernelNotConfiguredException", "DeviceSolver", "BatchSolver", "RecordingCallback", "PSTDError"]

@@ -17,8 +17,10 @@ namespace dev {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// Explicit FMAs (the device code is built without implicit contraction, so
+// the rounding is the same in every instantiation).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
 }
 // multiply by -i (forward) or +i (inverse)
 template <bool INV>
@@ -91,11 +93,17 @@ __host__ __device__ constexpr int fft_span(int F, int p) {  // product of radice
 
 // LDS address of element idx of pair p.  LAYOUT 0: pair-major (lanes vary the
 // element index); LAYOUT 1: element-major (lanes vary the pair index).  Both
-// make the natural-layout loads of a wavefront contiguous.
+// make the natural-layout loads of a wavefront contiguous.  One pad slot per 8
+// elements: the radix-8 passes write elements 8t + r, a 64 B lane stride that
+// would put every lane of a wavefront on the same few LDS banks.
+__host__ __device__ constexpr int lds_padded(int n) { return n + (n >> 3); }
 template <int F, int P, int LAYOUT>
 struct LdsMap {
-    static constexpr int SIZE = P * F;
-    __device__ __forceinline__ static int at(int p, int idx) { return LAYOUT == 0 ? p * F + idx : idx * P + p; }
+    static constexpr int SIZE = P * lds_padded(F);
+    __device__ __forceinline__ static int at(int p, int idx) {
+        const int e = idx + (idx >> 3);
+        return LAYOUT == 0 ? p * lds_padded(F) + e : e * P + p;
+    }
 };
 
 // Twiddle registers: pass p >= 1 needs (R_p - 1) factors per butterfly and
@@ -126,7 +134,11 @@ __device__ __forceinline__ void fft_load_twiddles(float2 *w, int t, const float2
 
 // One Stockham pass.  v: natural layout input.  Non-last passes write their
 // outputs to `out`; the last pass leaves them in v (natural layout).
-template <int F, int P, int LAYOUT, int PASS, bool INV>
+// TWG = false: w holds the thread's preloaded twiddle registers
+// (fft_load_twiddles); TWG = true: w is the twiddle table exp(-2 pi i k/F) in
+// global memory and each pass reads the factors it needs when it needs them
+// (fewer live registers, more waves per SIMD).
+template <int F, int P, int LAYOUT, int PASS, bool INV, bool TWG>
 __device__ __forceinline__ void fft_pass(float2 (&v)[8], int p, int t, float2 *out, const float2 *w) {
     constexpr int T = F / 8;
     constexpr int R = fft_radix(F, PASS);
@@ -145,7 +157,7 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[8], int p, int t, float2 *o
         if constexpr (NS > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                float2 wv = w[base_tw + s * (R - 1) + (r - 1)];
+                float2 wv = TWG ? w[(r * k) * (F / (NS * R))] : w[base_tw + s * (R - 1) + (r - 1)];
                 if (INV) wv.y = -wv.y;
                 a[r] = cmul(a[r], wv);
             }
@@ -172,138 +184,18 @@ __device__ __forceinline__ void lds_load(float2 (&v)[8], int p, int t, const flo
 
 // Full transform (forward if !INV) on natural-layout registers, through the two
 // LDS buffers.  Every thread of the block must call it (it synchronises).
-template <int F, int P, int LAYOUT, bool INV, int PASS = 0>
+template <int F, int P, int LAYOUT, bool INV, bool TWG, int PASS = 0>
 __device__ __forceinline__ void fft_run(float2 (&v)[8], int p, int t, float2 *buf0, float2 *buf1, const float2 *w) {
     constexpr int NP = fft_npasses(F);
     float2 *out = (PASS & 1) ? buf1 : buf0;
-    fft_pass<F, P, LAYOUT, PASS, INV>(v, p, t, out, w);
+    fft_pass<F, P, LAYOUT, PASS, INV, TWG>(v, p, t, out, w);
     if constexpr (PASS + 1 < NP) {
         __syncthreads();
         lds_load<F, P, LAYOUT>(v, p, t, out);
-        fft_run<F, P, LAYOUT, INV, PASS + 1>(v, p, t, buf0, buf1, w);
+        fft_run<F, P, LAYOUT, INV, TWG, PASS + 1>(v, p, t, buf0, buf1, w);
     }
 }
 
-
-// ---------------------------------------------------------------------------
-// Runtime-generic form: one code path for every F (the radix schedule, line
-// pairs per block and LDS layout are run-time values), so all workgroups of a
-// launch execute the same few KB of instructions.
-struct FFTGeom {
-    int F, T, P;        // length, threads per pair (F/8), pairs per block
-    int logT, logP;
-    int layout;         // 0 pair-major, 1 element-major LDS
-    int np;             // passes
-    int radix[5];
-    int span[5];
-};
-
-__device__ __forceinline__ FFTGeom fft_geom(int F, int NT, int layout) {
-    FFTGeom g;
-    g.F = F;
-    g.T = F >> 3;
-    g.P = NT / g.T;
-    g.logT = __ffs(g.T) - 1;
-    g.logP = __ffs(g.P) - 1;
-    g.layout = layout;
-    g.np = F == 128 ? 3 : F == 256 ? 3 : F == 512 ? 3 : F == 1024 ? 4 : F == 2048 ? 4 : F == 4096 ? 4 : 5;
-    int ns = 1;
-    for (int q = 0; q < 5; ++q) {
-        int r = 8;
-        if (q > 0 && q < g.np - 1) {
-            r = F == 128 ? 2 : F == 256 ? 4 : F == 512 ? 8 : F == 1024 ? 4 : F == 2048 ? (q == 1 ? 8 : 4)
-                : F == 4096 ? 8 : (q == 1 ? 2 : 8);
-        }
-        g.radix[q] = r;
-        g.span[q] = ns;
-        ns *= r;
-    }
-    return g;
-}
-
-__device__ __forceinline__ int lds_at(const FFTGeom &g, int p, int idx) {
-    return g.layout == 0 ? (p << (g.logT + 3)) + idx : (idx << g.logP) + p;
-}
-
-// Twiddles of pass q for thread t (at most 7 per thread).
-__device__ __forceinline__ void load_pass_twiddles(const FFTGeom &g, int q, int t, const float2 *__restrict__ tw,
-                                                   float2 (&w)[7]) {
-    const int R = g.radix[q], NS = g.span[q];
-    if (NS == 1) return;
-    const int stride = g.F / (NS * R);
-    if (R == 8) {
-        const int k = t & (NS - 1);
-#pragma unroll
-        for (int r = 1; r < 8; ++r) w[r - 1] = tw[r * k * stride];
-    } else if (R == 4) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int k = (t + s * g.T) & (NS - 1);
-#pragma unroll
-            for (int r = 1; r < 4; ++r) w[s * 3 + r - 1] = tw[r * k * stride];
-        }
-    } else {
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int k = (t + s * g.T) & (NS - 1);
-            w[s] = tw[k * stride];
-        }
-    }
-}
-
-template <int R, bool INV>
-__device__ __forceinline__ void rt_butterflies(const FFTGeom &g, int q, float2 (&v)[8], int p, int t, float2 *out,
-                                               const float2 (&w)[7], bool last) {
-    constexpr int NB = 8 / R;
-    const int NS = g.span[q];
-#pragma unroll
-    for (int s = 0; s < NB; ++s) {
-        const int j = t + s * g.T;
-        const int k = j & (NS - 1);
-        float2 a[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = v[s + r * NB];
-        if (NS > 1) {
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                float2 wv = w[s * (R - 1) + (r - 1)];
-                if (INV) wv.y = -wv.y;
-                a[r] = cmul(a[r], wv);
-            }
-        }
-        dft<R, INV>(a);
-        if (R == 8 && last) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[r] = a[r];
-        } else {
-            const int base = (j - k) * R + k;
-#pragma unroll
-            for (int r = 0; r < R; ++r) out[lds_at(g, p, base + r * NS)] = a[r];
-        }
-    }
-}
-
-// Full transform on natural-layout registers through LDS buffers buf0/buf1.
-// Twiddles of pass q+1 are fetched before the barrier that ends pass q.
-template <bool INV>
-__device__ __forceinline__ void fft_run_rt(const FFTGeom &g, float2 (&v)[8], int p, int t, float2 *buf0, float2 *buf1,
-                                        const float2 *__restrict__ tw) {
-    float2 w[7];
-    load_pass_twiddles(g, 0, t, tw, w);
-    for (int q = 0; q < g.np; ++q) {
-        const bool last = q == g.np - 1;
-        float2 *out = (q & 1) ? buf1 : buf0;
-        const int R = g.radix[q];
-        if (R == 8) rt_butterflies<8, INV>(g, q, v, p, t, out, w, last);
-        else if (R == 4) rt_butterflies<4, INV>(g, q, v, p, t, out, w, last);
-        else rt_butterflies<2, INV>(g, q, v, p, t, out, w, last);
-        if (last) break;
-        load_pass_twiddles(g, q + 1, t, tw, w);
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < 8; ++m) v[m] = out[lds_at(g, p, t + m * g.T)];
-    }
-}
 
 }  // namespace dev
 }  // namespace pstdamd

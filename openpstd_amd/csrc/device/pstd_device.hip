@@ -93,6 +93,7 @@ struct StageArgs {
     int nrec;
     float *samples;    // samples[slot * nrec + r], slot < 0: no sampling in this launch
     int slot;
+    int remap;         // XCD-contiguous tile order (tile_index)
 };
 
 // ------------------------------------------------------------- device code
@@ -200,7 +201,7 @@ __device__ __forceinline__ void gather_finish(const TileDesc &td, const TileGeom
         for (int q = 0; q < 2; ++q) {
             const float refl = stage[stage_at(g, 2 * p + q, w + mpos)];
             const float r = raw[m][q];
-            out[q] = mid ? r : ((left || right) ? (r * cs + refl * cm) * G.win[m] : 0.f);
+            out[q] = mid ? r : ((left || right) ? __builtin_fmaf(r, cs, refl * cm) * G.win[m] : 0.f);
         }
         v[m] = make_float2(out[0], out[1]);
     }
@@ -208,15 +209,22 @@ __device__ __forceinline__ void gather_finish(const TileDesc &td, const TileGeom
 
 // Forward transform, spectral derivative (the Hermitian filter h, which carries
 // the 1/F of kernel/core/kernel_functions.cpp:243,306), inverse transform.
-template <int F, int NT, int LAYOUT>
-__device__ __forceinline__ void derivative_fft(float2 (&v)[8], const float2 (&h)[8], const float2 *twr, int p, int t,
-                                               float2 *buf0, float2 *buf1) {
+// LEAN: the filter is read after the forward transform and twiddles are read
+// per pass from the table `tw` (instead of the preloaded registers `twr`).
+template <int F, int NT, int LAYOUT, bool LEAN>
+__device__ __forceinline__ void derivative_fft(float2 (&v)[8], float2 (&h)[8], const float2 *twr, int p, int t,
+                                               float2 *buf0, float2 *buf1, const float2 *hsrc, const float2 *tw) {
     constexpr int P = NT / (F / 8);
-    fft_run<F, P, LAYOUT, false>(v, p, t, buf0, buf1, twr);
+    const float2 *w = LEAN ? tw : twr;
+    fft_run<F, P, LAYOUT, false, LEAN>(v, p, t, buf0, buf1, w);
+    if constexpr (LEAN) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) h[m] = hsrc[t + m * (F / 8)];
+    }
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], h[m]);
     __syncthreads();
-    fft_run<F, P, LAYOUT, true>(v, p, t, buf0, buf1, twr);
+    fft_run<F, P, LAYOUT, true, LEAN>(v, p, t, buf0, buf1, w);
 }
 
 constexpr int kMaxTw = 28;  // fft_tw_total(F) for F <= 8192
@@ -268,7 +276,7 @@ __device__ __forceinline__ void finish_updates(const StageArgs &a, const TileDes
             const int id = u.idx[m][q];
             const float L = q ? v[m].y : v[m].x;
             const float upd = vel ? L * rho * c2 : L / rho;
-            const float nv = u.tnv[m][q] - s * upd;
+            const float nv = __builtin_fmaf(-s, upd, u.tnv[m][q]);
             if (id >= 0) {
                 a.out[id] = nv;
                 if (a.last) a.tn[id] = nv * u.pm[m][q];
@@ -278,41 +286,47 @@ __device__ __forceinline__ void finish_updates(const StageArgs &a, const TileDes
 }
 
 // One derivative tile: NT threads, F = td.F (8 values per thread and line pair).
-template <int NT, int FMIN, int FMAX>
+// !LEAN issues every load that depends only on the descriptor (filter,
+// twiddles, t_n and PML operands) before the transforms, for the shortest tile
+// latency; LEAN defers them to where they are used, for fewer live registers
+// and more resident waves.  Both compute the same values in the same order.
+template <int NT, int FMIN, int FMAX, bool LEAN>
 __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDesc &td, float2 *buf0, float2 *buf1) {
     const TileGeom g = tile_geom(td.F, NT, td.dir);
     const int tid = threadIdx.x;
     const int p = g.layout == 0 ? tid >> g.logT : tid & (g.P - 1);
     const int t = g.layout == 0 ? tid & (g.T - 1) : tid >> g.logP;
-    // loads in order of need: line data, filter and twiddles, update operands
     GatherRegs G;
     gather_issue(a, td, g, p, t, G);
     float2 h[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) h[m] = a.ctab[td.h_off + t + m * g.T];
-    float2 twr[kMaxTw];
+    float2 twr[LEAN ? 1 : kMaxTw];
     const float2 *tw = a.ctab + td.tw_off;
-    switch (td.F) {
-        case 128: if (FMIN <= 128 && FMAX >= 128) prefetch_twiddles<128>(twr, t, tw); break;
-        case 256: if (FMIN <= 256 && FMAX >= 256) prefetch_twiddles<256>(twr, t, tw); break;
-        case 512: if (FMIN <= 512 && FMAX >= 512) prefetch_twiddles<512>(twr, t, tw); break;
-        case 1024: if (FMIN <= 1024 && FMAX >= 1024) prefetch_twiddles<1024>(twr, t, tw); break;
-        case 2048: if (FMIN <= 2048 && FMAX >= 2048) prefetch_twiddles<2048>(twr, t, tw); break;
-        case 4096: if (FMIN <= 4096 && FMAX >= 4096) prefetch_twiddles<4096>(twr, t, tw); break;
-        case 8192: if (FMIN <= 8192 && FMAX >= 8192) prefetch_twiddles<8192>(twr, t, tw); break;
-        default: break;
+    const float2 *hsrc = a.ctab + td.h_off;
+    if constexpr (!LEAN) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) h[m] = hsrc[t + m * g.T];
+        switch (td.F) {
+            case 128: if (FMIN <= 128 && FMAX >= 128) prefetch_twiddles<128>(twr, t, tw); break;
+            case 256: if (FMIN <= 256 && FMAX >= 256) prefetch_twiddles<256>(twr, t, tw); break;
+            case 512: if (FMIN <= 512 && FMAX >= 512) prefetch_twiddles<512>(twr, t, tw); break;
+            case 1024: if (FMIN <= 1024 && FMAX >= 1024) prefetch_twiddles<1024>(twr, t, tw); break;
+            case 2048: if (FMIN <= 2048 && FMAX >= 2048) prefetch_twiddles<2048>(twr, t, tw); break;
+            case 4096: if (FMIN <= 4096 && FMAX >= 4096) prefetch_twiddles<4096>(twr, t, tw); break;
+            case 8192: if (FMIN <= 8192 && FMAX >= 8192) prefetch_twiddles<8192>(twr, t, tw); break;
+            default: break;
+        }
     }
     UpdateRegs u;
-    prefetch_updates(a, td, g, p, t, u);
+    if constexpr (!LEAN) prefetch_updates(a, td, g, p, t, u);
     float *stage = reinterpret_cast<float *>(buf1);
     float2 v[8];
     gather_finish(td, g, p, t, G, stage, v);
-#define PSTD_FFT_CASE(FV)                                                                           \
-    case FV:                                                                                        \
-        if constexpr (FMIN <= FV && FMAX >= FV) {                                                   \
-            if (g.layout == 0) derivative_fft<FV, NT, 0>(v, h, twr, p, t, buf0, buf1);              \
-            else derivative_fft<FV, NT, 1>(v, h, twr, p, t, buf0, buf1);                            \
-        }                                                                                           \
+#define PSTD_FFT_CASE(FV)                                                                               \
+    case FV:                                                                                            \
+        if constexpr (FMIN <= FV && FMAX >= FV) {                                                       \
+            if (g.layout == 0) derivative_fft<FV, NT, 0, LEAN>(v, h, twr, p, t, buf0, buf1, hsrc, tw);  \
+            else derivative_fft<FV, NT, 1, LEAN>(v, h, twr, p, t, buf0, buf1, hsrc, tw);                \
+        }                                                                                               \
         break;
     switch (td.F) {
         PSTD_FFT_CASE(128)
@@ -325,6 +339,7 @@ __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDe
         default: break;
     }
 #undef PSTD_FFT_CASE
+    if constexpr (LEAN) prefetch_updates(a, td, g, p, t, u);
     finish_updates(a, td, v, u);
 }
 
@@ -362,12 +377,25 @@ __device__ void passthrough_tile(const StageArgs &a, const TileDesc &td) {
     }
 }
 
-// two LDS buffers of P * F = 8 * NT complex values
-constexpr int lds_small_floats2() { return 2 * 8 * kSmallThreads; }
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8) and every XCD has its own L2.  With `remap`, XCD x works through a
+// contiguous run of the tile list instead of every 8th tile, so tiles of
+// neighbouring lines (the columns of one cache line, the rows of one halo)
+// share that L2.  A bijection of [0, n).
+__device__ __forceinline__ int tile_index(int b, int n, int remap) {
+    if (!remap) return b;
+    constexpr int X = 8;
+    const int x = b % X, l = b / X, q = n / X, r = n % X;
+    return x * q + min(x, r) + l;
+}
 
+// two LDS buffers of P * F = 8 * NT complex values each (padded)
+constexpr int kSmallBuf = lds_padded(8 * kSmallThreads);
+
+template <bool LEAN>
 __global__ void __launch_bounds__(kSmallThreads) stage_small_kernel(StageArgs a) {
-    __shared__ float2 lds[lds_small_floats2()];
-    const TileDesc &td = a.tiles[blockIdx.x];
+    __shared__ float2 lds[2 * kSmallBuf];
+    const TileDesc &td = a.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
     if (td.kind == 2) {
         sample_receivers(a);
         return;
@@ -376,14 +404,14 @@ __global__ void __launch_bounds__(kSmallThreads) stage_small_kernel(StageArgs a)
         passthrough_tile(a, td);
         return;
     }
-    derivative_tile<kSmallThreads, 128, 2048>(a, td, lds, lds + 8 * kSmallThreads);
+    derivative_tile<kSmallThreads, 128, 2048, LEAN>(a, td, lds, lds + kSmallBuf);
 }
 
-template <int F>
+template <int F, bool LEAN>
 __global__ void __launch_bounds__(F / 8) stage_big_kernel(StageArgs a) {
     extern __shared__ float2 lds_dyn[];
-    const TileDesc &td = a.tiles[blockIdx.x];
-    derivative_tile<F / 8, F, F>(a, td, lds_dyn, lds_dyn + F);
+    const TileDesc &td = a.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
+    derivative_tile<F / 8, F, F, LEAN>(a, td, lds_dyn, lds_dyn + LdsMap<F, 1, 0>::SIZE);
 }
 
 __global__ void receiver_kernel(const float *__restrict__ arena, const int2 *__restrict__ cells, int nrec,
@@ -450,12 +478,27 @@ struct pstd_ctx {
     int sample_cap = 0;
     int steps_done = 0;
     bool p0_explicit = true;
+    int xcd_remap = 1;
+    int lean = 0;  // stage kernel flavour (derivative_tile LEAN)
+    float tuned_stage_ms = 0;
+    // hipGraph of `graph_steps` whole time steps (0: launches are issued one by one)
+    int graph_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
 
     float *TN() const { return arena; }
     float *S(int i) const { return arena + (size_t)(1 + i) * arena_floats; }
     float *P0I() const { return arena + (size_t)3 * arena_floats; }
 
     void release() {
+        drop_graph();
         (void)hipFree(arena);
         (void)hipFree(pml);
         (void)hipFree(ctab);
@@ -552,20 +595,25 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
     a.s = ctx->dt * ctx->alpha[k];
     a.c2 = ctx->c2;
     a.last = k == 5;
+    a.remap = ctx->xcd_remap;
     const int ns = (int)ctx->tiles_small.size(), n4 = (int)ctx->tiles_4096.size(), n8 = (int)ctx->tiles_8192.size();
+    const bool lean = ctx->lean != 0;
     if (ns) {
         a.tiles = ctx->d_tiles;
-        hipLaunchKernelGGL(stage_small_kernel, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
+        if (lean) hipLaunchKernelGGL(stage_small_kernel<true>, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
+        else hipLaunchKernelGGL(stage_small_kernel<false>, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
     }
     if (n4) {
         a.tiles = ctx->d_tiles + ns;
         size_t lds = 2 * sizeof(float2) * LdsMap<4096, 1, 0>::SIZE;
-        hipLaunchKernelGGL(stage_big_kernel<4096>, dim3(n4), dim3(512), lds, ctx->stream, a);
+        if (lean) hipLaunchKernelGGL((stage_big_kernel<4096, true>), dim3(n4), dim3(512), lds, ctx->stream, a);
+        else hipLaunchKernelGGL((stage_big_kernel<4096, false>), dim3(n4), dim3(512), lds, ctx->stream, a);
     }
     if (n8) {
         a.tiles = ctx->d_tiles + ns + n4;
         size_t lds = 2 * sizeof(float2) * LdsMap<8192, 1, 0>::SIZE;
-        hipLaunchKernelGGL(stage_big_kernel<8192>, dim3(n8), dim3(1024), lds, ctx->stream, a);
+        if (lean) hipLaunchKernelGGL((stage_big_kernel<8192, true>), dim3(n8), dim3(1024), lds, ctx->stream, a);
+        else hipLaunchKernelGGL((stage_big_kernel<8192, false>), dim3(n8), dim3(1024), lds, ctx->stream, a);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -624,11 +672,13 @@ int pstd_ctx_create(int device, pstd_ctx **out) {
         delete ctx;
         return PSTD_ERR_DEVICE;
     }
-    // the F = 8192 class needs 128 KiB of LDS per workgroup
-    if (hipFuncSetAttribute((const void *)stage_big_kernel<8192>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            2 * sizeof(float2) * 8192) != hipSuccess ||
-        hipFuncSetAttribute((const void *)stage_big_kernel<4096>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            2 * sizeof(float2) * 4096) != hipSuccess) {
+    // the long-line classes need more than 64 KiB of LDS per workgroup
+    const size_t lds8 = 2 * sizeof(float2) * LdsMap<8192, 1, 0>::SIZE, lds4 = 2 * sizeof(float2) * LdsMap<4096, 1, 0>::SIZE;
+    const hipFuncAttribute attr = hipFuncAttributeMaxDynamicSharedMemorySize;
+    if (hipFuncSetAttribute((const void *)stage_big_kernel<8192, false>, attr, lds8) != hipSuccess ||
+        hipFuncSetAttribute((const void *)stage_big_kernel<8192, true>, attr, lds8) != hipSuccess ||
+        hipFuncSetAttribute((const void *)stage_big_kernel<4096, false>, attr, lds4) != hipSuccess ||
+        hipFuncSetAttribute((const void *)stage_big_kernel<4096, true>, attr, lds4) != hipSuccess) {
         ctx->err = "cannot reserve LDS for the long-line kernels";
     }
     *out = ctx;
@@ -646,8 +696,68 @@ void pstd_ctx_destroy(pstd_ctx *ctx) {
 
 const char *pstd_ctx_last_error(const pstd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
-    if (!ctx || !plan) return PSTD_ERR_INVALID;
+// Enqueues `chunk` time steps: 6 stage launches per step (the first stage of
+// step s also samples step s-1 into sample slot s-1) and the sampling of the
+// chunk's last step into slot chunk-1.
+static int enqueue_steps(pstd_ctx *ctx, int chunk) {
+    for (int s = 0; s < chunk; ++s) {
+        for (int k = 0; k < 6; ++k) {
+            int rc = launch_stage(ctx, k, s - 1);
+            if (rc) return rc;
+            if (k == 0) ctx->p0_explicit = false;
+        }
+    }
+    if (ctx->nrec > 0) {
+        hipLaunchKernelGGL(receiver_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->S(1), ctx->d_cells, ctx->nrec,
+                           ctx->d_samples, chunk - 1);
+        HIP_TRY(hipGetLastError());
+    }
+    return PSTD_OK;
+}
+
+// Launch-shape autotuner.  Times one stage-kernel flavour x tile order per
+// candidate on the freshly uploaded (all-zero) state and keeps the fastest.
+// Zero fields stay exactly zero under the update, and the arenas are cleared
+// again afterwards, so tuning leaves no trace in the results.
+static int autotune(pstd_ctx *ctx) {
+    static const int cand[4][2] = {{0, 1}, {1, 1}, {0, 0}, {1, 0}};  // (lean, xcd order)
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 0;
+    int bi = 0, rc = PSTD_OK;
+    for (int c = 0; c < 4 && rc == PSTD_OK; ++c) {
+        ctx->lean = cand[c][0];
+        ctx->xcd_remap = cand[c][1];
+        rc = enqueue_steps(ctx, 1);
+        float ms = 0;
+        if (rc == PSTD_OK && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = PSTD_ERR_DEVICE;
+        if (rc == PSTD_OK) rc = enqueue_steps(ctx, 3);
+        if (rc == PSTD_OK && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = PSTD_ERR_DEVICE;
+        if (rc == PSTD_OK && hipEventSynchronize(e1) != hipSuccess) rc = PSTD_ERR_DEVICE;
+        if (rc == PSTD_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PSTD_ERR_DEVICE;
+        if (rc == PSTD_OK && (c == 0 || ms < best)) {
+            best = ms;
+            bi = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        if (ctx->err.empty()) ctx->err = "autotune: HIP event failure";
+        return rc;
+    }
+    ctx->lean = cand[bi][0];
+    ctx->xcd_remap = cand[bi][1];
+    ctx->tuned_stage_ms = best / 18.0;
+    HIP_TRY(hipMemsetAsync(ctx->arena, 0, 4 * (size_t)ctx->arena_floats * sizeof(float), ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->steps_done = 0;
+    ctx->p0_explicit = true;
+    return PSTD_OK;
+}
+
+static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->release();
@@ -919,7 +1029,66 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     ctx->steps_done = 0;
     ctx->p0_explicit = true;
     ctx->planned = true;
-    return PSTD_OK;
+    return autotune(ctx);
+}
+
+int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
+    if (!ctx || !plan) return PSTD_ERR_INVALID;
+    return upload_impl(ctx, plan);
+}
+
+int pstd_ctx_upload_plans(pstd_ctx *ctx, const pstd_plan_desc *plans, int n_plans) {
+    if (!ctx || !plans || n_plans <= 0) return PSTD_ERR_INVALID;
+    if (n_plans == 1) return upload_impl(ctx, plans);
+    // independent scenes: one plan with the domain lists concatenated; scenes
+    // share the time step, the RK coefficients and the grid (one launch per
+    // stage advances all of them)
+    std::vector<pstd_plan_domain> doms;
+    std::vector<pstd_plan_segment> segs;
+    std::vector<pstd_plan_receiver> rcv;
+    const pstd_plan_desc &p0 = plans[0];
+    for (int i = 0; i < n_plans; ++i) {
+        const pstd_plan_desc &p = plans[i];
+        bool same = p.dx == p0.dx && p.dt == p0.dt && p.c1 == p0.c1 && p.patch_error == p0.patch_error;
+        for (int k = 0; k < 6; ++k) same = same && p.rk_alpha[k] == p0.rk_alpha[k];
+        if (!same) {
+            ctx->err = "batched scenes must share dx, dt, c1, patch error and RK coefficients (scene " +
+                       std::to_string(i) + " differs)";
+            return PSTD_ERR_INVALID;
+        }
+        if (p.n_domains < 0 || p.n_segments < 0 || p.n_receivers < 0) return PSTD_ERR_INVALID;
+        const int base = (int)doms.size();
+        for (int d = 0; d < p.n_domains; ++d) doms.push_back(p.domains[d]);
+        for (int k = 0; k < p.n_segments; ++k) {
+            pstd_plan_segment s = p.segments[k];
+            const bool bad = s.domain < 0 || s.domain >= p.n_domains || s.side1 >= p.n_domains || s.side2 >= p.n_domains;
+            if (bad) {
+                ctx->err = "segment of scene " + std::to_string(i) + " names a domain outside its scene";
+                return PSTD_ERR_INVALID;
+            }
+            s.domain += base;
+            if (s.side1 >= 0) s.side1 += base;
+            if (s.side2 >= 0) s.side2 += base;
+            segs.push_back(s);
+        }
+        for (int r = 0; r < p.n_receivers; ++r) {
+            pstd_plan_receiver c = p.receivers[r];
+            if (c.domain < 0 || c.domain >= p.n_domains) {
+                ctx->err = "receiver of scene " + std::to_string(i) + " outside its scene";
+                return PSTD_ERR_INVALID;
+            }
+            c.domain += base;
+            rcv.push_back(c);
+        }
+    }
+    pstd_plan_desc m = p0;
+    m.n_domains = (int)doms.size();
+    m.domains = doms.data();
+    m.n_segments = (int)segs.size();
+    m.segments = segs.data();
+    m.n_receivers = (int)rcv.size();
+    m.receivers = rcv.data();
+    return upload_impl(ctx, &m);
 }
 
 int pstd_ctx_set_field(pstd_ctx *ctx, int domain, int field, const float *host, size_t count) {
@@ -980,27 +1149,47 @@ int pstd_ctx_read_frame(pstd_ctx *ctx, int domain, float *out, size_t count) {
     return pstd_ctx_get_field(ctx, domain, PSTD_FIELD_P0, out, count);
 }
 
+// Captures enqueue_steps(graph_steps) into a graph.  Only valid once p0 comes
+// from the stage outputs (after the first step), which every replay keeps.
+static int build_graph(pstd_ctx *ctx) {
+    ctx->drop_graph();
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_steps(ctx, ctx->graph_steps);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        ctx->err = std::string("graph capture: ") + hipGetErrorString(e);
+        return PSTD_ERR_DEVICE;
+    }
+    ctx->graph = g;
+    HIP_TRY(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+    return PSTD_OK;
+}
+
 static int advance_impl(pstd_ctx *ctx, int steps, float *samples) {
     if (!ctx || !ctx->planned) return PSTD_ERR_INVALID;
     if (steps < 0) return PSTD_ERR_INVALID;
     HIP_TRY(hipSetDevice(ctx->device));
     int done = 0;
     while (done < steps) {
-        const int chunk = std::min(steps - done, ctx->sample_cap);
-        for (int s = 0; s < chunk; ++s) {
-            for (int k = 0; k < 6; ++k) {
-                // the first stage of step s samples step s-1 of this chunk
-                int rc = launch_stage(ctx, k, s - 1);
+        int chunk = std::min(steps - done, ctx->sample_cap);
+        const bool use_graph = ctx->graph_steps > 0 && !ctx->p0_explicit && chunk >= ctx->graph_steps;
+        if (use_graph) {
+            chunk = ctx->graph_steps;
+            if (!ctx->graph_exec) {
+                int rc = build_graph(ctx);
                 if (rc) return rc;
-                if (k == 0) ctx->p0_explicit = false;
             }
-            ctx->steps_done++;
+            HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+        } else {
+            int rc = enqueue_steps(ctx, chunk);
+            if (rc) return rc;
         }
-        if (ctx->nrec > 0) {  // the chunk's last step
-            hipLaunchKernelGGL(receiver_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->S(1), ctx->d_cells, ctx->nrec,
-                               ctx->d_samples, chunk - 1);
-            HIP_TRY(hipGetLastError());
-        }
+        ctx->steps_done += chunk;
         if (samples && ctx->nrec > 0) {
             HIP_TRY(hipMemcpyAsync(samples + (size_t)done * ctx->nrec, ctx->d_samples,
                                    (size_t)chunk * ctx->nrec * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
@@ -1028,6 +1217,45 @@ int pstd_ctx_synchronize(pstd_ctx *ctx) {
 }
 
 int pstd_ctx_steps_done(const pstd_ctx *ctx) { return ctx ? ctx->steps_done : -1; }
+
+int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps) {
+    if (!ctx || steps < 0) return PSTD_ERR_INVALID;
+    if (ctx->planned && steps > ctx->sample_cap) {
+        ctx->err = "graph longer than the sample buffer (" + std::to_string(ctx->sample_cap) + " steps)";
+        return PSTD_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graph();
+    ctx->graph_steps = steps;
+    return PSTD_OK;
+}
+
+int pstd_ctx_set_kernel_flavour(pstd_ctx *ctx, int lean) {
+    if (!ctx) return PSTD_ERR_INVALID;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graph();
+    ctx->lean = lean ? 1 : 0;
+    return PSTD_OK;
+}
+
+int pstd_ctx_get_tuning(const pstd_ctx *ctx, int32_t *lean, int32_t *xcd_contiguous, double *stage_ms) {
+    if (!ctx || !ctx->planned) return PSTD_ERR_INVALID;
+    if (lean) *lean = ctx->lean;
+    if (xcd_contiguous) *xcd_contiguous = ctx->xcd_remap;
+    if (stage_ms) *stage_ms = ctx->tuned_stage_ms;
+    return PSTD_OK;
+}
+
+int pstd_ctx_set_tile_order(pstd_ctx *ctx, int xcd_contiguous) {
+    if (!ctx) return PSTD_ERR_INVALID;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graph();
+    ctx->xcd_remap = xcd_contiguous ? 1 : 0;
+    return PSTD_OK;
+}
 
 int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int *launches) {
     if (!ctx || !ctx->planned || steps <= 0 || !mean_stage_ms) return PSTD_ERR_INVALID;

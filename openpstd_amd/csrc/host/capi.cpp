@@ -2,6 +2,7 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <vector>
 
 #include "pstd_kernel.h"
 
@@ -212,6 +213,43 @@ int pstd_kernel_prepare(pstd_kernel *k, pstd_ctx **ctx) {
         *ctx = k->k->prepare_solver()->context();
         return PSTD_OK;
     });
+}
+
+int pstd_batch_prepare(pstd_kernel *const *kernels, int n, int device, pstd_ctx **out) {
+    if (!kernels || n <= 0 || !out) return PSTD_ERR_INVALID;
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!kernels[i]) return PSTD_ERR_INVALID;
+        if (!kernels[i]->k->configured()) return PSTD_ERR_NOT_CONFIGURED;
+        if (!kernels[i]->k->gpu()) return PSTD_ERR_UNSUPPORTED;
+    }
+    std::vector<pstd_plan_desc> plans(n);
+    std::vector<std::vector<pstd_plan_domain>> doms(n);
+    std::vector<std::vector<pstd_plan_receiver>> rcv(n);
+    for (int i = 0; i < n; ++i) kernels[i]->k->get_scene().plan(&plans[i], doms[i], rcv[i]);
+    pstd_ctx *ctx = nullptr;
+    int rc = pstd_ctx_create(device, &ctx);
+    if (rc) return rc;
+    rc = pstd_ctx_upload_plans(ctx, plans.data(), n);
+    int base = 0;
+    for (int i = 0; i < n && rc == PSTD_OK; ++i) {
+        const pstdamd::SceneModel &s = kernels[i]->k->get_scene();
+        for (int d = 0; d < s.n_domains() && rc == PSTD_OK; ++d) {
+            const pstdamd::DomainModel &m = s.domain(d);
+            rc = pstd_ctx_set_field(ctx, base + d, PSTD_FIELD_P0, m.p0.data(), m.p0.size());
+            if (!rc) rc = pstd_ctx_set_field(ctx, base + d, PSTD_FIELD_PX, m.px0.data(), m.px0.size());
+            if (!rc) rc = pstd_ctx_set_field(ctx, base + d, PSTD_FIELD_PY, m.py0.data(), m.py0.size());
+        }
+        base += s.n_domains();
+    }
+    if (rc) {
+        // the message goes with the first kernel (the context is gone)
+        kernels[0]->err = std::string("pstd_batch_prepare: ") + pstd_ctx_last_error(ctx);
+        pstd_ctx_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return PSTD_OK;
 }
 
 int pstd_kernel_segment_count(pstd_kernel *k) {

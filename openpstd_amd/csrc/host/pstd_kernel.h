@@ -44,6 +44,7 @@ public:
     SimulationMetadata get_metadata() override;
     const pstdamd::SceneModel &get_scene() const { return scene; }
     bool configured() const { return (bool)config; }
+    bool gpu() const { return GPU; }
     // solver of the last run (its device state stays readable until the next run)
     const GPUSolver *last_solver() const { return solver.get(); }
     // a solver with its device state initialised (frame 0 next), not run

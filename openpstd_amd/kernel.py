@@ -176,24 +176,16 @@ class PSTDKernel(KernelInterface):
         return out
 
 
-class DeviceSolver:
-    """The device context of a configured PSTDKernel, driven step by step.
+class _Context:
+    """Shared driver of a device context (pstd_ctx_*)."""
 
-    ``PSTDKernel.prepare()`` uploads the plan and the initial fields (frame 0
-    next); ``advance(n)`` runs n time steps and returns the receiver samples
-    (n x receivers), one per step as openPSTD's Receiver::received_values.
-    """
-
-    def __init__(self, kernel):
-        self.kernel = kernel  # owns the context
-        self.ctx = C.c_void_p()
-        _check(lib().pstd_kernel_prepare(kernel._h, C.byref(self.ctx)), kernel._err)
-        self.n_receivers = len(kernel._config.Receivers)
+    n_receivers = 0
 
     def _err(self):
         return lib().pstd_ctx_last_error(self.ctx).decode()
 
     def advance(self, steps):
+        """Runs `steps` time steps; returns the receiver samples (steps x receivers)."""
         out = np.zeros((steps, max(1, self.n_receivers)), dtype=np.float32)
         ptr = out.ctypes.data_as(C.POINTER(C.c_float)) if self.n_receivers else None
         _check(lib().pstd_ctx_advance(self.ctx, steps, ptr), self._err)
@@ -222,16 +214,100 @@ class DeviceSolver:
         _check(lib().pstd_ctx_profile_stages(self.ctx, steps, C.byref(ms), C.byref(n)), self._err)
         return ms.value, n.value
 
-    def field(self, i, name):
-        out = np.zeros(self.kernel._shape(i, name), dtype=np.float32)
+    def set_graph_steps(self, steps):
+        """Replay blocks of `steps` time steps as one captured hipGraph (0: off)."""
+        _check(lib().pstd_ctx_set_graph_steps(self.ctx, steps), self._err)
+
+    def set_tile_order(self, xcd_contiguous):
+        _check(lib().pstd_ctx_set_tile_order(self.ctx, int(bool(xcd_contiguous))), self._err)
+
+    def set_kernel_flavour(self, lean):
+        _check(lib().pstd_ctx_set_kernel_flavour(self.ctx, int(bool(lean))), self._err)
+
+    def tuning(self):
+        """The autotuner's choice: {"lean", "xcd_order", "stage_ms"}."""
+        lean, xcd = C.c_int32(), C.c_int32()
+        ms = C.c_double()
+        _check(lib().pstd_ctx_get_tuning(self.ctx, C.byref(lean), C.byref(xcd), C.byref(ms)), self._err)
+        return {"lean": lean.value, "xcd_order": xcd.value, "stage_ms": ms.value}
+
+    def _get(self, i, name, shape):
+        out = np.zeros(shape, dtype=np.float32)
         _check(lib().pstd_ctx_get_field(self.ctx, i, T.FIELD_IDS[name], out.ctypes.data_as(C.POINTER(C.c_float)),
                                         out.size), self._err)
         return out
 
-    def set_field(self, i, name, values):
+    def _set(self, i, name, values):
         v = np.ascontiguousarray(values, dtype=np.float32)
         _check(lib().pstd_ctx_set_field(self.ctx, i, T.FIELD_IDS[name], v.ctypes.data_as(C.POINTER(C.c_float)),
                                         v.size), self._err)
+
+
+class DeviceSolver(_Context):
+    """The device context of a configured PSTDKernel, driven step by step.
+
+    ``PSTDKernel.prepare()`` uploads the plan and the initial fields (frame 0
+    next); ``advance(n)`` runs n time steps and returns the receiver samples
+    (n x receivers), one per step as openPSTD's Receiver::received_values.
+    """
+
+    def __init__(self, kernel):
+        self.kernel = kernel  # owns the context
+        self.ctx = C.c_void_p()
+        _check(lib().pstd_kernel_prepare(kernel._h, C.byref(self.ctx)), kernel._err)
+        self.n_receivers = len(kernel._config.Receivers)
+
+    def field(self, i, name):
+        return self._get(i, name, self.kernel._shape(i, name))
+
+    def set_field(self, i, name, values):
+        self._set(i, name, values)
+
+
+class BatchSolver(_Context):
+    """Independent scenes advanced together: one launch per RK stage covers all
+    of them (pstd_batch_prepare).  Every scene evolves exactly as it would alone.
+
+        b = BatchSolver([conf0, conf1, ...])
+        samples = b.advance(n)          # n x (receivers of scene 0, scene 1, ...)
+        b.field(scene, domain, "p0")
+    """
+
+    def __init__(self, configs, device=0):
+        self.kernels = []
+        for conf in configs:
+            k = PSTDKernel(GPU=True)
+            k.initialize_kernel(conf)
+            self.kernels.append(k)
+        self.domain_base, self.receiver_base = [], []
+        nd = nr = 0
+        for k, conf in zip(self.kernels, configs):
+            self.domain_base.append(nd)
+            self.receiver_base.append(nr)
+            nd += k.domain_count()
+            nr += len(conf.Receivers)
+        self.n_receivers = nr
+        handles = (C.c_void_p * len(self.kernels))(*[k._h.value for k in self.kernels])
+        self.ctx = C.c_void_p()
+        _check(lib().pstd_batch_prepare(handles, len(self.kernels), device, C.byref(self.ctx)), self.kernels[0]._err)
+
+    def __del__(self):
+        ctx = getattr(self, "ctx", None)
+        if ctx and ctx.value:
+            lib().pstd_ctx_destroy(ctx)
+            self.ctx = None
+
+    def scene_samples(self, samples, scene):
+        """Columns of `samples` that belong to `scene`."""
+        b = self.receiver_base[scene]
+        n = len(self.kernels[scene]._config.Receivers)
+        return samples[:, b:b + n]
+
+    def field(self, scene, domain, name):
+        return self._get(self.domain_base[scene] + domain, name, self.kernels[scene]._shape(domain, name))
+
+    def set_field(self, scene, domain, name, values):
+        self._set(self.domain_base[scene] + domain, name, values)
 
 
 class RecordingCallback(KernelCallback):

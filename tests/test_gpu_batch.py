@@ -1,0 +1,122 @@
+"""Batched scenes, hipGraph replay and tile order on the GPU.
+
+A batch advances every scene with the same per-line arithmetic as a single-
+scene context, so results must be bit-identical to running the scenes one by
+one; graph replay and the XCD tile order change only how launches are issued
+and which workgroup takes which tile, so they must be bit-identical too.  One
+scene of the batch is also checked against the FP64 oracle (same tolerances as
+tests/test_gpu_parity.py).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import openpstd_amd as P
+from openpstd_amd import workloads as W
+from oracle import binding as ob
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def single(conf):
+    k = P.PSTDKernel(GPU=True)
+    k.initialize_kernel(conf)
+    return k, k.prepare()
+
+
+def test_batch_equals_scenes_run_alone():
+    steps = 30
+    confs = W.scene_batch(7, steps, n_scenes=5, hi=300)
+    print("batch of", len(confs), "scenes,", steps, "steps")
+    b = P.BatchSolver(confs)
+    got = b.advance(steps)
+    assert got.shape == (steps, b.n_receivers)
+    for i, conf in enumerate(confs):
+        k, dev = single(conf)
+        want = dev.advance(steps)
+        assert np.array_equal(b.scene_samples(got, i), want), i
+        for d in range(k.domain_count()):
+            for f in ("p0", "px0", "py0", "vx0", "vy0"):
+                assert np.array_equal(b.field(i, d, f), dev.field(d, f)), (i, d, f)
+
+
+def test_batch_scene_against_oracle():
+    steps = 40
+    confs = W.scene_batch(3, steps, n_scenes=3, hi=260)
+    b = P.BatchSolver(confs)
+    got = b.advance(steps)
+    for i in (0, 2):
+        want = ob.OracleScene(confs[i], 64, THREADS).step(steps)
+        g = b.scene_samples(got, i)
+        assert np.abs(g - want).max() <= 2e-6
+        if np.abs(want).max() > 1e-2:
+            assert rel(g, want) <= 1e-4
+
+
+def test_batch_rejects_mixed_time_steps():
+    a = W.room(10)
+    c = W.room(10)
+    c.Settings.SetFactRK(0.4)
+    with pytest.raises(P.PSTDError):
+        P.BatchSolver([a, c])
+
+
+@pytest.mark.parametrize("graph_steps", [1, 7])
+def test_graph_replay_is_bitwise_eager(graph_steps):
+    """Replays with eager work, host reads and synchronisations in between."""
+    conf = W.lshape(1, 80)
+    _, eager = single(conf)
+    k, g = single(conf)
+    g.set_graph_steps(graph_steps)
+    for n in (1, 20, 3, 30, 26):
+        print("advance", n, "graph", graph_steps)
+        a = eager.advance(n)
+        b = g.advance(n)
+        assert np.array_equal(a, b)
+        for d in range(k.domain_count()):
+            assert np.array_equal(eager.field(d, "p0"), g.field(d, "p0")), d
+    p = g.field(0, "p0") * 0.5
+    eager.set_field(0, "p0", p)
+    g.set_field(0, "p0", p)
+    assert np.array_equal(eager.advance(9), g.advance(9))
+    assert np.array_equal(eager.field(1, "vx0"), g.field(1, "vx0"))
+
+
+@pytest.mark.parametrize("shape", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_launch_shape_is_bitwise_neutral(shape):
+    """Every (kernel flavour, tile order) the autotuner may pick gives the same bits."""
+    conf = W.lshape(2, 25)
+    k, a = single(conf)
+    _, b = single(conf)
+    print("autotuned:", a.tuning())
+    a.set_kernel_flavour(False)
+    a.set_tile_order(False)
+    b.set_kernel_flavour(shape[0])
+    b.set_tile_order(shape[1])
+    assert np.array_equal(a.advance(25), b.advance(25))
+    for d in range(k.domain_count()):
+        for f in ("p0", "vx0", "py0"):
+            assert np.array_equal(a.field(d, f), b.field(d, f)), (d, f)
+
+
+def test_long_lines_lean_flavour_bitwise():
+    """F = 8192 lines in both flavours of the long-line kernel."""
+    conf = P.PSTDConfiguration()
+    W.base_settings(conf, 3)
+    conf.Domains = [W.domain(0, 0, 4100, 40, 1.0)]
+    conf.Speakers = [(W._world_pos(2050), W._world_pos(20.0), 0.0)]
+    k, a = single(conf)
+    _, b = single(conf)
+    a.set_kernel_flavour(False)
+    b.set_kernel_flavour(True)
+    a.advance(3)
+    b.advance(3)
+    for d in range(k.domain_count()):
+        assert np.array_equal(a.field(d, "p0"), b.field(d, "p0")), d
