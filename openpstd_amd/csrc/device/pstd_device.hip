@@ -43,10 +43,15 @@ namespace {
 
 constexpr int kSmallThreads = 256;  // block size of the F <= 2048 class
 constexpr int kPassChunk = 2048;    // elements per passthrough tile
+constexpr int kMaxDomainFloats = 1 << 29;  // one domain block (4 fields) per buffer resource
 
+// A domain's four fields (vx, vy, px, py, row-major) form one block at the same
+// offset of every arena (TN, S0, S1, P0I and the PML factors).
 struct DomainDev {
     int nx, ny;
-    int off_vx, off_vy, off_px, off_py;
+    long long base;                 // element offset of the block in an arena
+    int size;                       // elements of the block
+    int f_vx, f_vy, f_px, f_py;     // field offsets within the block
     float rho;
     int is_pml;
 };
@@ -57,39 +62,48 @@ struct DomainDev {
 //          kernel/core/Domain.cpp:149-370): gather, derivative, update.
 //  kind 1: passthrough — elements [line0, line0+count) of one field that no
 //          segment updates (rigid domains, directions openPSTD skips).
-// Source value of a domain at (line, pos):  A[a + line*ls + pos*ps]
-// (+ B[b + same] for p0 = px0 + py0).
+//  kind 2: receiver sampling (first stage of a step).
+// Memory is addressed through buffer resources over one domain block (base
+// `*_dom`, `*_size` elements): value (line, pos) of a source field is element
+// a + line*ls + pos*ps of the block (+ element b of the second array for
+// p0 = px0 + py0).  Offsets past the block read 0 and drop stores, which is how
+// absent neighbours, the zero padding and partial tiles are expressed.
 struct TileDesc {
     int kind;
     int F;
     int line0, count;
-    int wlen, n_main, n_out, vel, dir;
+    int wlen, wlog, n_main, n_out, vel, dir;  // wlog: log2 of the halo slot (2^wlog >= wlen)
     int h_off, tw_off, win_off;
+    long long m_dom, s1_dom, s2_dom;
+    int m_size, s1_size, s2_size;
     int m_a, m_b, m_ls, m_ps;
     int s1_a, s1_b, s1_ls, s1_ps, s1_line, s1_len;  // s1_len < 0: no neighbour (zeros)
     int s2_a, s2_b, s2_ls, s2_ps, s2_line, s2_len;
     float c_s1, c_m1, c_s2, c_m2;
-    int o_off, o_ls, o_ps;  // output field (vx, px, vy or py of the domain)
-    float rho;
+    int o_f, o_ls, o_ps;  // output field (vx, px, vy or py) within the main block
+    float coef;           // 1/rho (velocity update) or rho*c1^2 (pressure update)
     int is_pml;
+};
+
+struct ReceiverCell {
+    long long px, py;  // arena element offsets of the cell's px0 and py0
 };
 
 struct StageArgs {
     const TileDesc *__restrict__ tiles;
     const float2 *ctab;
     const float *ftab;
-    const float *p0a;  // p0 = p0a[a + idx] + (p0b ? p0b[b + idx] : 0)
+    const float *p0a;  // p0 = p0a[...] + (p0b ? p0b[...] : 0)
     const float *p0b;
     const float *vsrc; // current vx / vy
     float *tn;
     float *out;
     const float *pml;  // PML factor arena (same layout); 1 for non-PML domains
     float s;           // dt * alpha_k
-    float c2;          // c1 * c1
     int last;          // last RK stage: also write tn = out * pml
     // receiver sampling of the previous step (first stage of a step): p0 = px + py
     // of the previous step's raw stage-5 output (kernel/Solver.cpp:86-91)
-    const int2 *cells;
+    const ReceiverCell *cells;
     int nrec;
     float *samples;    // samples[slot * nrec + r], slot < 0: no sampling in this launch
     int slot;
@@ -97,6 +111,20 @@ struct StageArgs {
 };
 
 // ------------------------------------------------------------- device code
+
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int kOOB = (int)0x80000000u;  // byte offset past every block: loads 0, drops stores
+
+__device__ __forceinline__ Rsrc block_rsrc(const float *arena, long long dom, int size) {
+    // raw buffer (stride 0) over `size` floats; word 3 as for every gfx9 raw buffer
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(arena + dom), (short)0, size * 4, 0x00020000);
+}
+__device__ __forceinline__ float bload(Rsrc r, int off_bytes) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off_bytes, 0, 0));
+}
+__device__ __forceinline__ void bstore(Rsrc r, int off_bytes, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off_bytes, 0, 0);
+}
 
 // Per-tile geometry shared by the generic (F-independent) parts of a tile.
 struct TileGeom {
@@ -114,96 +142,144 @@ __device__ __forceinline__ TileGeom tile_geom(int F, int NT, int layout) {
     return g;
 }
 
-// The windowed lines of one tile (spatderp3, kernel/core/kernel_functions.cpp:
-// 210-216 pressure, :263-269 velocity):  [dom1 | main | dom3 | 0], where
-// dom1[j] = (side1[tail + j] * c_s1 + main[reflected] * c_m1) * window[j] and
-// dom3 likewise.  Thread (p, t) loads exactly the raw values it will transform,
-// positions t + m*T of lines 2p and 2p+1, branch-free (every load issued from a
-// valid address, masked afterwards) so all its loads are in flight together.
-// The raw values go through LDS only so that halo positions can read the
-// reflected main values staged by other threads.
+// Raw staging of the lines in LDS (float, line-major for rows, position-major
+// for columns so that the wavefront's lanes stay contiguous).
 __device__ __forceinline__ int stage_at(const TileGeom &g, int ll, int pos) {
     return g.layout == 0 ? (ll << (g.logT + 3)) + pos : (pos << (g.logP + 1)) + ll;
 }
 
+// The windowed lines of one tile (spatderp3, kernel/core/kernel_functions.cpp:
+// 210-216 pressure, :263-269 velocity):  [dom1 | main | dom3 | 0], where
+// dom1[j] = (side1[tail + j] * c_s1 + main[reflected] * c_m1) * window[j] and
+// dom3 likewise.  Thread (p, t) loads the main values it will transform
+// (positions t + m*T of lines 2p and 2p+1); the 2 x w halo values of every
+// line are loaded by the tile's threads in turn; both go to LDS so that halo
+// positions can combine them with the reflected main values.  All loads are
+// issued before the first wait.
 struct GatherRegs {
-    float ra[8][2], rb[8][2], win[8];
-    bool okv[8][2];
-    bool hasB;
+    float ra[8][2], rb[8][2];
+    float h1a[4], h1b[4], h2a[4], h2b[4];
+    int hpos1[4], hpos2[4];
 };
 
-// Issues the gather loads of thread (p, t) (no waits).
+template <int NT>
 __device__ __forceinline__ void gather_issue(const StageArgs &a, const TileDesc &td, const TileGeom &g, int p, int t,
                                              GatherRegs &G) {
     const int w = td.wlen, nm = td.n_main, vel = td.vel, count = td.count;
-    const int m_a = td.m_a, m_b = td.m_b, m_ls = td.m_ls, m_ps = td.m_ps;
-    const int s1_a = td.s1_a, s1_b = td.s1_b, s1_ls = td.s1_ls, s1_ps = td.s1_ps, s1_line = td.s1_line;
-    const int s2_a = td.s2_a, s2_b = td.s2_b, s2_ls = td.s2_ls, s2_ps = td.s2_ps, s2_line = td.s2_line;
-    const int s1_len = td.s1_len, s2_len = td.s2_len, win_off = td.win_off;
-    const float c_s1 = td.c_s1, c_m1 = td.c_m1, c_s2 = td.c_s2, c_m2 = td.c_m2;
     const float *A = vel ? a.vsrc : a.p0a;
     const bool hasB = !vel && a.p0b != nullptr;
     const float *B = hasB ? a.p0b : A;
-    G.hasB = hasB;
-    const int line_a = td.line0 + 2 * p;
-    const bool ok0 = 2 * p < count, ok1 = 2 * p + 1 < count;
+    // resources: main, side1, side2 blocks; B shifted so that A's offsets address it
+    const Rsrc rmA = block_rsrc(A, td.m_dom, td.m_size);
+    const Rsrc rmB = block_rsrc(B, td.m_dom + (td.m_b - td.m_a), td.m_size - (td.m_b - td.m_a));
+    const int la = td.line0 + 2 * p;
+    const int lbase = (td.m_a + la * td.m_ls) * 4, lstep = td.m_ls * 4, pstep = td.m_ps * 4;
+    int off[8][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        const int pos = t + (m << g.logT);
-        const bool left = pos < w, right = pos >= w + nm, mid = !left && !right;
-        const int ba = left ? s1_a : (right ? s2_a : m_a);
-        const int bb = left ? s1_b : (right ? s2_b : m_b);
-        const int ls = left ? s1_ls : (right ? s2_ls : m_ls);
-        const int ps = left ? s1_ps : (right ? s2_ps : m_ps);
-        const int lo = left ? s1_line : (right ? s2_line : 0);
-        const int sp = left ? s1_len - w - vel + pos : (right ? pos - w - nm + vel : pos - w);
-        const bool vr = mid || (left ? s1_len >= 0 : (s2_len >= 0 && pos < 2 * w + nm));
-        const int i0 = (line_a + lo) * ls + sp * ps;
-        const bool v0 = vr && ok0, v1 = vr && ok1;
-        const int x0 = v0 ? ba + i0 : m_a, x1 = v1 ? ba + i0 + ls : m_a;
-        const int y0 = v0 && hasB ? bb + i0 : m_a, y1 = v1 && hasB ? bb + i0 + ls : m_a;
-        G.ra[m][0] = A[x0];
-        G.ra[m][1] = A[x1];
-        G.rb[m][0] = B[y0];
-        G.rb[m][1] = B[y1];
-        G.okv[m][0] = v0;
-        G.okv[m][1] = v1;
-        G.win[m] = a.ftab[win_off + (left ? pos : ((right && pos < 2 * w + nm) ? pos - nm + 1 : 0))];
+        const int e = t + (m << g.logT) - w;
+        const bool ok = e >= 0 && e < nm;
+        off[m][0] = ok && 2 * p < count ? lbase + e * pstep : kOOB;
+        off[m][1] = ok && 2 * p + 1 < count ? lbase + lstep + e * pstep : kOOB;
+    }
+    if (hasB) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                G.ra[m][q] = bload(rmA, off[m][q]);
+                G.rb[m][q] = bload(rmB, off[m][q]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                G.ra[m][q] = bload(rmA, off[m][q]);
+                G.rb[m][q] = 0.f;
+            }
+        }
+    }
+    // halos: item i of a side is (line ll, offset j), j in slots of 2^wlog >= w;
+    // rows: j fastest, columns: ll fastest (coalesced either way)
+    const int n_items = (2 * g.P) << td.wlog;
+    const int nk = (n_items + NT - 1) / NT;
+    const Rsrc r1A = block_rsrc(A, td.s1_dom, td.s1_size);
+    const Rsrc r1B = block_rsrc(B, td.s1_dom + (td.s1_b - td.s1_a), td.s1_size - (td.s1_b - td.s1_a));
+    const Rsrc r2A = block_rsrc(A, td.s2_dom, td.s2_size);
+    const Rsrc r2B = block_rsrc(B, td.s2_dom + (td.s2_b - td.s2_a), td.s2_size - (td.s2_b - td.s2_a));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        G.hpos1[k] = G.hpos2[k] = -1;
+        G.h1a[k] = G.h1b[k] = G.h2a[k] = G.h2b[k] = 0.f;
+        if (k < nk) {
+            const int i = threadIdx.x + k * NT;
+            const int ll = g.layout == 0 ? i >> td.wlog : i & (2 * g.P - 1);
+            const int j = g.layout == 0 ? i & ((1 << td.wlog) - 1) : i >> (g.logP + 1);
+            const bool item = i < n_items && j < w;
+            const bool ok = item && ll < count;
+            const int line = td.line0 + ll;
+            const int o1 = ok && td.s1_len >= 0
+                               ? (td.s1_a + (line + td.s1_line) * td.s1_ls + (td.s1_len - w - vel + j) * td.s1_ps) * 4
+                               : kOOB;
+            const int o2 = ok && td.s2_len >= 0 ? (td.s2_a + (line + td.s2_line) * td.s2_ls + (j + vel) * td.s2_ps) * 4
+                                                : kOOB;
+            G.h1a[k] = bload(r1A, o1);
+            G.h2a[k] = bload(r2A, o2);
+            if (hasB) {
+                G.h1b[k] = bload(r1B, o1);
+                G.h2b[k] = bload(r2B, o2);
+            }
+            G.hpos1[k] = item ? stage_at(g, ll, j) : -1;
+            G.hpos2[k] = item ? stage_at(g, ll, w + nm + j) : -1;
+        }
     }
 }
 
 // Stages the raw values, then builds the windowed complex pairs.
-__device__ __forceinline__ void gather_finish(const TileDesc &td, const TileGeom &g, int p, int t, const GatherRegs &G,
-                                              float *stage, float2 (&v)[8]) {
+__device__ __forceinline__ void gather_finish(const StageArgs &a, const TileDesc &td, const TileGeom &g, int p, int t,
+                                              const GatherRegs &G, float *stage, float *win, float2 (&v)[8]) {
     const int w = td.wlen, nm = td.n_main, vel = td.vel;
     const float c_s1 = td.c_s1, c_m1 = td.c_m1, c_s2 = td.c_s2, c_m2 = td.c_m2;
-    const bool hasB = G.hasB;
     float raw[8][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         const int pos = t + (m << g.logT);
+        const bool mid = pos >= w && pos < w + nm;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            raw[m][q] = G.okv[m][q] ? (hasB ? G.ra[m][q] + G.rb[m][q] : G.ra[m][q]) : 0.f;
-            stage[stage_at(g, 2 * p + q, pos)] = raw[m][q];
+            raw[m][q] = G.ra[m][q] + G.rb[m][q];
+            if (mid) stage[stage_at(g, 2 * p + q, pos)] = raw[m][q];
         }
     }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (G.hpos1[k] >= 0) {
+            stage[G.hpos1[k]] = G.h1a[k] + G.h1b[k];
+            stage[G.hpos2[k]] = G.h2a[k] + G.h2b[k];
+        }
+    }
+    if ((int)threadIdx.x <= 2 * w) win[threadIdx.x] = a.ftab[td.win_off + threadIdx.x];
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         const int i = t + (m << g.logT);
         const bool left = i < w, right = i >= w + nm && i < 2 * w + nm, mid = i >= w && i < w + nm;
         const int j = i - w - nm;
-        const int mpos = left ? (vel ? w - i : w - 1 - i) : (right ? (vel ? nm - 2 - j : nm - 1 - j) : 0);
-        const float cs = left ? c_s1 : c_s2, cm = left ? c_m1 : c_m2;
-        float out[2];
+        float out[2] = {0.f, 0.f};
+        if (left || right) {
+            const int mpos = left ? (vel ? w - i : w - 1 - i) : (vel ? nm - 2 - j : nm - 1 - j);
+            const float cs = left ? c_s1 : c_s2, cm = left ? c_m1 : c_m2;
+            const float wv = win[left ? i : w + 1 + j];
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const float refl = stage[stage_at(g, 2 * p + q, w + mpos)];
-            const float r = raw[m][q];
-            out[q] = mid ? r : ((left || right) ? __builtin_fmaf(r, cs, refl * cm) * G.win[m] : 0.f);
+            for (int q = 0; q < 2; ++q) {
+                const float sv = stage[stage_at(g, 2 * p + q, i)];
+                const float refl = stage[stage_at(g, 2 * p + q, w + mpos)];
+                out[q] = __builtin_fmaf(sv, cs, refl * cm) * wv;
+            }
         }
-        v[m] = make_float2(out[0], out[1]);
+        v[m] = make_float2(mid ? raw[m][0] : out[0], mid ? raw[m][1] : out[1]);
     }
 }
 
@@ -235,52 +311,49 @@ __device__ __forceinline__ void prefetch_twiddles(float2 *twr, int t, const floa
     fft_load_twiddles<F>(twr, t, tw);
 }
 
-// Solver::update_field_values (kernel/Solver.cpp:174-181).  The t_n values
-// (and, in the last stage, the PML factors) depend only on the descriptor, so
-// they are loaded at the start of the tile and stay in registers.
+// Solver::update_field_values (kernel/Solver.cpp:174-181): out = t_n - s*L*coef,
+// the last stage also t_n = out * PML factor.  Positions outside [0, n_out)
+// and lines past the tile get an out-of-block offset (no store).
 struct UpdateRegs {
-    int idx[8][2];
+    int off[8][2];
     float tnv[8][2], pm[8][2];
 };
 
 __device__ __forceinline__ void prefetch_updates(const StageArgs &a, const TileDesc &td, const TileGeom &g, int p,
                                                  int t, UpdateRegs &u) {
-    const int w = td.wlen, n_out = td.n_out, o_off = td.o_off, o_ls = td.o_ls, o_ps = td.o_ps;
+    const int w = td.wlen, n_out = td.n_out;
     const bool ok0 = 2 * p < td.count, ok1 = 2 * p + 1 < td.count;
-    const int base = o_off + (td.line0 + 2 * p) * o_ls;
+    const int base = (td.o_f + (td.line0 + 2 * p) * td.o_ls) * 4, lstep = td.o_ls * 4, pstep = td.o_ps * 4;
     const bool use_pml = a.last && td.is_pml;
+    const Rsrc rt = block_rsrc(a.tn, td.m_dom, td.m_size);
+    const Rsrc rp = block_rsrc(a.pml, td.m_dom, td.m_size);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         const int pos = t + (m << g.logT) - w;
         const bool okp = pos >= 0 && pos < n_out;
-        const int i0 = base + pos * o_ps;
-        u.idx[m][0] = okp && ok0 ? i0 : -1;
-        u.idx[m][1] = okp && ok1 ? i0 + o_ls : -1;
+        u.off[m][0] = okp && ok0 ? base + pos * pstep : kOOB;
+        u.off[m][1] = okp && ok1 ? base + lstep + pos * pstep : kOOB;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const int id = u.idx[m][q] >= 0 ? u.idx[m][q] : o_off;
-            u.tnv[m][q] = a.tn[id];
-            u.pm[m][q] = use_pml ? a.pml[id] : 1.f;
+            u.tnv[m][q] = bload(rt, u.off[m][q]);
+            u.pm[m][q] = use_pml ? bload(rp, u.off[m][q]) : 1.f;
         }
     }
 }
 
 __device__ __forceinline__ void finish_updates(const StageArgs &a, const TileDesc &td, const float2 (&v)[8],
                                                const UpdateRegs &u) {
-    const float rho = td.rho, c2 = a.c2, s = a.s;
-    const bool vel = td.vel != 0;
+    const float sc = a.s * td.coef;
+    const Rsrc ro = block_rsrc(a.out, td.m_dom, td.m_size);
+    const Rsrc rt = block_rsrc(a.tn, td.m_dom, td.m_size);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const int id = u.idx[m][q];
             const float L = q ? v[m].y : v[m].x;
-            const float upd = vel ? L * rho * c2 : L / rho;
-            const float nv = __builtin_fmaf(-s, upd, u.tnv[m][q]);
-            if (id >= 0) {
-                a.out[id] = nv;
-                if (a.last) a.tn[id] = nv * u.pm[m][q];
-            }
+            const float nv = __builtin_fmaf(-sc, L, u.tnv[m][q]);
+            bstore(ro, u.off[m][q], nv);
+            if (a.last) bstore(rt, u.off[m][q], nv * u.pm[m][q]);
         }
     }
 }
@@ -297,7 +370,7 @@ __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDe
     const int p = g.layout == 0 ? tid >> g.logT : tid & (g.P - 1);
     const int t = g.layout == 0 ? tid & (g.T - 1) : tid >> g.logP;
     GatherRegs G;
-    gather_issue(a, td, g, p, t, G);
+    gather_issue<NT>(a, td, g, p, t, G);
     float2 h[8];
     float2 twr[LEAN ? 1 : kMaxTw];
     const float2 *tw = a.ctab + td.tw_off;
@@ -318,9 +391,11 @@ __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDe
     }
     UpdateRegs u;
     if constexpr (!LEAN) prefetch_updates(a, td, g, p, t, u);
+    // buf1 holds the raw lines (2P x F floats) and, after them, the window
     float *stage = reinterpret_cast<float *>(buf1);
+    float *win = stage + 16 * NT;
     float2 v[8];
-    gather_finish(td, g, p, t, G, stage, v);
+    gather_finish(a, td, g, p, t, G, stage, win, v);
 #define PSTD_FFT_CASE(FV)                                                                               \
     case FV:                                                                                            \
         if constexpr (FMIN <= FV && FMAX >= FV) {                                                       \
@@ -348,8 +423,8 @@ __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDe
 __device__ void sample_receivers(const StageArgs &a) {
     if (a.slot < 0) return;
     for (int r = threadIdx.x; r < a.nrec; r += blockDim.x) {
-        const int2 c = a.cells[r];
-        a.samples[(size_t)a.slot * a.nrec + r] = a.p0a[c.x] + a.p0b[c.y];
+        const ReceiverCell c = a.cells[r];
+        a.samples[(size_t)a.slot * a.nrec + r] = a.p0a[c.px] + a.p0b[c.py];
     }
 }
 
@@ -358,22 +433,23 @@ __device__ void sample_receivers(const StageArgs &a) {
 // thread issues all of its loads before its stores.
 __device__ void passthrough_tile(const StageArgs &a, const TileDesc &td) {
     constexpr int PER = kPassChunk / kSmallThreads;
+    const Rsrc rt = block_rsrc(a.tn, td.m_dom, td.m_size);
+    const Rsrc rp = block_rsrc(a.pml, td.m_dom, td.m_size);
+    const Rsrc ro = block_rsrc(a.out, td.m_dom, td.m_size);
+    const bool use_pml = a.last && td.is_pml;
     float v[PER], f[PER];
+    int off[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int e = threadIdx.x + k * kSmallThreads;
-        const bool ok = e < td.count;
-        const int idx = td.o_off + td.line0 + e;
-        v[k] = ok ? a.tn[idx] : 0.f;
-        f[k] = (ok && a.last && td.is_pml) ? a.pml[idx] : 1.f;
+        off[k] = e < td.count ? (td.o_f + td.line0 + e) * 4 : kOOB;
+        v[k] = bload(rt, off[k]);
+        f[k] = use_pml ? bload(rp, off[k]) : 1.f;
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const int e = threadIdx.x + k * kSmallThreads;
-        if (e >= td.count) continue;
-        const int idx = td.o_off + td.line0 + e;
-        a.out[idx] = v[k];
-        if (a.last && td.is_pml) a.tn[idx] = v[k] * f[k];
+        bstore(ro, off[k], v[k]);
+        if (use_pml) bstore(rt, off[k], v[k] * f[k]);
     }
 }
 
@@ -395,7 +471,8 @@ constexpr int kSmallBuf = lds_padded(8 * kSmallThreads);
 template <bool LEAN>
 __global__ void __launch_bounds__(kSmallThreads) stage_small_kernel(StageArgs a) {
     __shared__ float2 lds[2 * kSmallBuf];
-    const TileDesc &td = a.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
+    // a copy taken before any store: the descriptor stays in scalar registers
+    const TileDesc td = a.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
     if (td.kind == 2) {
         sample_receivers(a);
         return;
@@ -410,26 +487,31 @@ __global__ void __launch_bounds__(kSmallThreads) stage_small_kernel(StageArgs a)
 template <int F, bool LEAN>
 __global__ void __launch_bounds__(F / 8) stage_big_kernel(StageArgs a) {
     extern __shared__ float2 lds_dyn[];
-    const TileDesc &td = a.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
+    const TileDesc td = a.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
     derivative_tile<F / 8, F, F, LEAN>(a, td, lds_dyn, lds_dyn + LdsMap<F, 1, 0>::SIZE);
 }
 
-__global__ void receiver_kernel(const float *__restrict__ arena, const int2 *__restrict__ cells, int nrec,
+__global__ void receiver_kernel(const float *__restrict__ arena, const ReceiverCell *__restrict__ cells, int nrec,
                                 float *__restrict__ samples, int slot) {
-    // cells[r] = (px slot index, py slot index); p0 = px0 + py0 (kernel/Solver.cpp:114-116)
+    // p0 = px0 + py0 (kernel/Solver.cpp:114-116)
     for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
-        int2 c = cells[r];
-        samples[(size_t)slot * nrec + r] = arena[c.x] + arena[c.y];
+        const ReceiverCell c = cells[r];
+        samples[(size_t)slot * nrec + r] = arena[c.px] + arena[c.py];
     }
 }
 
-__global__ void p0_materialise_kernel(const float *__restrict__ src, float *__restrict__ p0i, const int4 *ranges,
-                                      int nr) {
-    // ranges[k] = (px offset, py offset, count, -)
+struct P0Range {
+    long long px, py;  // arena element offsets of px0 and py0 of one domain
+    long long n;
+};
+
+__global__ void p0_materialise_kernel(const float *__restrict__ src, float *__restrict__ p0i,
+                                      const P0Range *__restrict__ ranges, int nr) {
+    // P0I.px-slot = px0 + py0 of every domain
     for (int k = blockIdx.y; k < nr; k += gridDim.y) {
-        int4 r = ranges[k];
-        for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < r.z; e += gridDim.x * blockDim.x)
-            p0i[r.x + e] = src[r.x + e] + src[r.y + e];
+        const P0Range r = ranges[k];
+        for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < r.n; e += (long long)gridDim.x * blockDim.x)
+            p0i[r.px + e] = src[r.px + e] + src[r.py + e];
     }
 }
 
@@ -461,8 +543,8 @@ struct pstd_ctx {
     std::vector<DomainDev> doms;
     std::vector<pstd_plan_domain> plan_domains;
     std::vector<TileDesc> tiles_small, tiles_4096, tiles_8192;
-    int arena_floats = 0;
-    float dt = 0, c2 = 0;
+    size_t arena_floats = 0;
+    float dt = 0, c1 = 0;
     float alpha[6] = {0};
     int nrec = 0;
     // device buffers
@@ -471,8 +553,8 @@ struct pstd_ctx {
     float2 *ctab = nullptr;
     float *ftab = nullptr;
     TileDesc *d_tiles = nullptr;  // small | 4096 | 8192
-    int2 *d_cells = nullptr;
-    int4 *d_ranges = nullptr;
+    ReceiverCell *d_cells = nullptr;
+    P0Range *d_ranges = nullptr;
     int n_ranges = 0;
     float *d_samples = nullptr;
     int sample_cap = 0;
@@ -494,8 +576,8 @@ struct pstd_ctx {
     }
 
     float *TN() const { return arena; }
-    float *S(int i) const { return arena + (size_t)(1 + i) * arena_floats; }
-    float *P0I() const { return arena + (size_t)3 * arena_floats; }
+    float *S(int i) const { return arena + (1 + i) * arena_floats; }
+    float *P0I() const { return arena + 3 * arena_floats; }
 
     void release() {
         drop_graph();
@@ -593,7 +675,6 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
     a.out = ctx->S(outi);
     a.pml = ctx->pml;
     a.s = ctx->dt * ctx->alpha[k];
-    a.c2 = ctx->c2;
     a.last = k == 5;
     a.remap = ctx->xcd_remap;
     const int ns = (int)ctx->tiles_small.size(), n4 = (int)ctx->tiles_4096.size(), n8 = (int)ctx->tiles_8192.size();
@@ -631,13 +712,13 @@ int field_size(const pstd_plan_domain &d, int f) {
     }
 }
 
-int field_off(const DomainDev &d, int f) {
+// arena element offset of a field (p0 lives in the px slot of P0I)
+long long field_off(const DomainDev &d, int f) {
     switch (f) {
-        case PSTD_FIELD_VX: return d.off_vx;
-        case PSTD_FIELD_VY: return d.off_vy;
-        case PSTD_FIELD_PX: return d.off_px;
-        case PSTD_FIELD_PY: return d.off_py;
-        default: return d.off_px;  // p0 lives in the px slot of P0I
+        case PSTD_FIELD_VX: return d.base + d.f_vx;
+        case PSTD_FIELD_VY: return d.base + d.f_vy;
+        case PSTD_FIELD_PY: return d.base + d.f_py;
+        default: return d.base + d.f_px;
     }
 }
 
@@ -766,37 +847,42 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     ctx->tiles_4096.clear();
     ctx->tiles_8192.clear();
     ctx->plan_domains.assign(plan->domains, plan->domains + plan->n_domains);
-    // arena layout
-    int off = 0;
-    auto align = [](int v) { return (v + 63) & ~63; };
+    // arena layout: one block per domain (vx, vy, px, py), 256 B aligned
+    long long off = 0;
+    auto align = [](long long v) { return (v + 63) & ~63LL; };
     for (int i = 0; i < plan->n_domains; ++i) {
         const pstd_plan_domain &p = plan->domains[i];
         if (p.nx <= 0 || p.ny <= 0) {
             ctx->err = "domain with empty extent";
             return PSTD_ERR_INVALID;
         }
+        const long long cells = (long long)p.nx * p.ny, block = 4 * cells + p.nx + p.ny + 4 * 64;
+        if (block > kMaxDomainFloats) {
+            ctx->err = "domain of " + std::to_string(cells) + " cells exceeds one buffer block";
+            return PSTD_ERR_UNSUPPORTED;
+        }
         DomainDev d{};
         d.nx = p.nx;
         d.ny = p.ny;
         d.rho = p.rho;
         d.is_pml = p.is_pml;
-        d.off_vx = off;
-        off = align(off + p.ny * (p.nx + 1));
-        d.off_vy = off;
-        off = align(off + (p.ny + 1) * p.nx);
-        d.off_px = off;
-        off = align(off + p.ny * p.nx);
-        d.off_py = off;
-        off = align(off + p.ny * p.nx);
-        if (off < 0 || off > (1 << 30)) {
-            ctx->err = "scene too large for 32-bit arena offsets";
-            return PSTD_ERR_UNSUPPORTED;
-        }
+        d.base = off;
+        long long f = 0;
+        d.f_vx = (int)f;
+        f = align(f + (long long)p.ny * (p.nx + 1));
+        d.f_vy = (int)f;
+        f = align(f + (long long)(p.ny + 1) * p.nx);
+        d.f_px = (int)f;
+        f = align(f + cells);
+        d.f_py = (int)f;
+        f = align(f + cells);
+        d.size = (int)f;
+        off += f;
         ctx->doms.push_back(d);
     }
-    ctx->arena_floats = std::max(off, 64);
+    ctx->arena_floats = (size_t)std::max(off, 64LL);
     ctx->dt = plan->dt;
-    ctx->c2 = plan->c1 * plan->c1;
+    ctx->c1 = plan->c1;
     for (int k = 0; k < 6; ++k) ctx->alpha[k] = plan->rk_alpha[k];
     // device plan cache: filters, twiddles, windows
     std::vector<float2> ctab;
@@ -807,20 +893,19 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     const double dx = plan->dx;
     // source addressing of a domain's derivative input for (direction, field):
     // pressure reads p0 = px + py (row stride nx), velocity vx (X, stride nx+1) or vy (Y)
+    // (field offsets within the domain block; velocity lines have no second array)
     auto src_addr = [&](const DomainDev &d, int dir, int field, int &a, int &b, int &ls, int &ps) {
         if (field == 0) {
-            a = d.off_px;
-            b = d.off_py;
+            a = d.f_px;
+            b = d.f_py;
             ls = dir == 0 ? d.nx : 1;
             ps = dir == 0 ? 1 : d.nx;
         } else if (dir == 0) {
-            a = d.off_vx;
-            b = -1;
+            a = b = d.f_vx;
             ls = d.nx + 1;
             ps = 1;
         } else {
-            a = d.off_vy;
-            b = -1;
+            a = b = d.f_vy;
             ls = 1;
             ps = d.nx;
         }
@@ -829,9 +914,13 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     for (int si = 0; si < plan->n_segments; ++si) {
         const pstd_plan_segment &s = plan->segments[si];
         if (s.domain < 0 || s.domain >= plan->n_domains || s.side1 >= plan->n_domains || s.side2 >= plan->n_domains ||
-            (s.direction != 0 && s.direction != 1) || (s.field != 0 && s.field != 1) || s.wlen <= 0 || s.wlen > 60) {
+            (s.direction != 0 && s.direction != 1) || (s.field != 0 && s.field != 1) || s.wlen <= 0) {
             ctx->err = "invalid segment " + std::to_string(si);
             return PSTD_ERR_INVALID;
+        }
+        if (s.wlen > 32) {
+            ctx->err = "window half-length " + std::to_string(s.wlen) + " above 32 (patch error above ~72 dB)";
+            return PSTD_ERR_UNSUPPORTED;
         }
         const DomainDev &dm = ctx->doms[s.domain];
         const int n = s.direction == 0 ? dm.nx : dm.ny;
@@ -852,8 +941,14 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         g.c_m1 = s.c_main1;
         g.c_s2 = s.c_side2;
         g.c_m2 = s.c_main2;
-        g.rho = dm.rho;
+        // update factor of Solver::update_field_values (kernel/Solver.cpp:174-181),
+        // in double: 1/rho for the velocity fields, rho*c1^2 for the pressure fields
+        g.coef = s.field == 0 ? (float)(1.0 / (double)dm.rho) : (float)((double)dm.rho * plan->c1 * plan->c1);
         g.is_pml = dm.is_pml;
+        g.wlog = 0;
+        while ((1 << g.wlog) < s.wlen) ++g.wlog;
+        g.m_dom = g.s1_dom = g.s2_dom = dm.base;
+        g.m_size = g.s1_size = g.s2_size = dm.size;
         if (g.F < 128 || g.F > 8192) {
             ctx->err = "FFT length " + std::to_string(g.F) + " outside 128..8192";
             return PSTD_ERR_UNSUPPORTED;
@@ -884,19 +979,23 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
                 src_addr(dn, s.direction, s.field, g.s1_a, g.s1_b, g.s1_ls, g.s1_ps);
                 g.s1_line = o;
                 g.s1_len = nlen;
+                g.s1_dom = dn.base;
+                g.s1_size = dn.size;
             } else {
                 src_addr(dn, s.direction, s.field, g.s2_a, g.s2_b, g.s2_ls, g.s2_ps);
                 g.s2_line = o;
                 g.s2_len = nlen;
+                g.s2_dom = dn.base;
+                g.s2_size = dn.size;
             }
         }
         // output field: X pressure -> vx, X velocity -> px, Y pressure -> vy, Y velocity -> py
         if (s.direction == 0) {
-            g.o_off = s.field == 0 ? dm.off_vx : dm.off_px;
+            g.o_f = s.field == 0 ? dm.f_vx : dm.f_px;
             g.o_ls = s.field == 0 ? dm.nx + 1 : dm.nx;
             g.o_ps = 1;
         } else {
-            g.o_off = s.field == 0 ? dm.off_vy : dm.off_py;
+            g.o_f = s.field == 0 ? dm.f_vy : dm.f_py;
             g.o_ls = 1;
             g.o_ps = dm.nx;
         }
@@ -942,7 +1041,7 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     // openPSTD does not update).  covered[] index: 0 px, 1 py, 2 vx, 3 vy
     for (int i = 0; i < plan->n_domains; ++i) {
         const DomainDev &d = ctx->doms[i];
-        const int offs[4] = {d.off_px, d.off_py, d.off_vx, d.off_vy};
+        const int offs[4] = {d.f_px, d.f_py, d.f_vx, d.f_vy};
         const int sizes[4] = {d.nx * d.ny, d.nx * d.ny, d.ny * (d.nx + 1), (d.ny + 1) * d.nx};
         for (int f = 0; f < 4; ++f) {
             if (covered[i][f]) continue;
@@ -951,7 +1050,9 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
                 t.kind = 1;
                 t.line0 = e;
                 t.count = std::min(kPassChunk, sizes[f] - e);
-                t.o_off = offs[f];
+                t.m_dom = d.base;
+                t.m_size = d.size;
+                t.o_f = offs[f];
                 t.is_pml = d.is_pml;
                 ctx->tiles_small.push_back(t);
             }
@@ -965,13 +1066,13 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     if (ctab.empty()) ctab.push_back(make_float2(0, 0));
     if (ftab.empty()) ftab.push_back(0);
     // PML factor arena (1 where no PML)
-    std::vector<float> pml((size_t)ctx->arena_floats, 1.f);
+    std::vector<float> pml(ctx->arena_floats, 1.f);
     for (int i = 0; i < plan->n_domains; ++i) {
         const pstd_plan_domain &p = plan->domains[i];
         if (!p.is_pml) continue;
         const DomainDev &d = ctx->doms[i];
         const float *src[4] = {p.pml_vx, p.pml_vy, p.pml_px, p.pml_py};
-        const int offs[4] = {d.off_vx, d.off_vy, d.off_px, d.off_py};
+        const long long offs[4] = {d.base + d.f_vx, d.base + d.f_vy, d.base + d.f_px, d.base + d.f_py};
         const int sizes[4] = {field_size(p, PSTD_FIELD_VX), field_size(p, PSTD_FIELD_VY), field_size(p, PSTD_FIELD_PX),
                               field_size(p, PSTD_FIELD_PY)};
         for (int f = 0; f < 4; ++f) {
@@ -983,7 +1084,7 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         }
     }
     // receivers
-    std::vector<int2> cells;
+    std::vector<ReceiverCell> cells;
     for (int r = 0; r < plan->n_receivers; ++r) {
         const pstd_plan_receiver &rc = plan->receivers[r];
         if (rc.domain < 0 || rc.domain >= plan->n_domains || rc.index < 0 ||
@@ -992,14 +1093,14 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
             return PSTD_ERR_INVALID;
         }
         const DomainDev &d = ctx->doms[rc.domain];
-        cells.push_back(make_int2(d.off_px + rc.index, d.off_py + rc.index));
+        cells.push_back({d.base + d.f_px + rc.index, d.base + d.f_py + rc.index});
     }
     ctx->nrec = (int)cells.size();
-    std::vector<int4> ranges;
-    for (const DomainDev &d : ctx->doms) ranges.push_back(make_int4(d.off_px, d.off_py, d.nx * d.ny, 0));
+    std::vector<P0Range> ranges;
+    for (const DomainDev &d : ctx->doms) ranges.push_back({d.base + d.f_px, d.base + d.f_py, (long long)d.nx * d.ny});
     ctx->n_ranges = (int)ranges.size();
     // allocate + upload
-    const size_t af = (size_t)ctx->arena_floats;
+    const size_t af = ctx->arena_floats;
     HIP_TRY(hipMalloc(&ctx->arena, 4 * af * sizeof(float)));
     HIP_TRY(hipMemsetAsync(ctx->arena, 0, 4 * af * sizeof(float), ctx->stream));
     HIP_TRY(hipMalloc(&ctx->pml, af * sizeof(float)));
@@ -1015,13 +1116,13 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     if (!all.empty())
         HIP_TRY(hipMemcpyAsync(ctx->d_tiles, all.data(), all.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
                                ctx->stream));
-    HIP_TRY(hipMalloc(&ctx->d_cells, std::max<size_t>(1, cells.size()) * sizeof(int2)));
+    HIP_TRY(hipMalloc(&ctx->d_cells, std::max<size_t>(1, cells.size()) * sizeof(ReceiverCell)));
     if (!cells.empty())
-        HIP_TRY(hipMemcpyAsync(ctx->d_cells, cells.data(), cells.size() * sizeof(int2), hipMemcpyHostToDevice,
+        HIP_TRY(hipMemcpyAsync(ctx->d_cells, cells.data(), cells.size() * sizeof(ReceiverCell), hipMemcpyHostToDevice,
                                ctx->stream));
-    HIP_TRY(hipMalloc(&ctx->d_ranges, std::max<size_t>(1, ranges.size()) * sizeof(int4)));
+    HIP_TRY(hipMalloc(&ctx->d_ranges, std::max<size_t>(1, ranges.size()) * sizeof(P0Range)));
     if (!ranges.empty())
-        HIP_TRY(hipMemcpyAsync(ctx->d_ranges, ranges.data(), ranges.size() * sizeof(int4), hipMemcpyHostToDevice,
+        HIP_TRY(hipMemcpyAsync(ctx->d_ranges, ranges.data(), ranges.size() * sizeof(P0Range), hipMemcpyHostToDevice,
                                ctx->stream));
     ctx->sample_cap = 4096;
     HIP_TRY(hipMalloc(&ctx->d_samples, (size_t)ctx->sample_cap * std::max(1, ctx->nrec) * sizeof(float)));
@@ -1104,7 +1205,7 @@ int pstd_ctx_set_field(pstd_ctx *ctx, int domain, int field, const float *host, 
     if (field == PSTD_FIELD_P0) {
         int rc = materialise_p0(ctx);
         if (rc) return rc;
-        dst = ctx->P0I() + ctx->doms[domain].off_px;
+        dst = ctx->P0I() + field_off(ctx->doms[domain], PSTD_FIELD_PX);
     } else {
         dst = ctx->TN() + field_off(ctx->doms[domain], field);
     }
@@ -1125,14 +1226,14 @@ int pstd_ctx_get_field(pstd_ctx *ctx, int domain, int field, float *host, size_t
     const DomainDev &d = ctx->doms[domain];
     if (field == PSTD_FIELD_P0) {
         if (ctx->p0_explicit) {
-            HIP_TRY(hipMemcpyAsync(host, ctx->P0I() + d.off_px, count * sizeof(float), hipMemcpyDeviceToHost,
+            HIP_TRY(hipMemcpyAsync(host, ctx->P0I() + field_off(d, PSTD_FIELD_PX), count * sizeof(float), hipMemcpyDeviceToHost,
                                    ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
         } else {
             std::vector<float> py(count);
-            HIP_TRY(hipMemcpyAsync(host, ctx->S(1) + d.off_px, count * sizeof(float), hipMemcpyDeviceToHost,
+            HIP_TRY(hipMemcpyAsync(host, ctx->S(1) + field_off(d, PSTD_FIELD_PX), count * sizeof(float), hipMemcpyDeviceToHost,
                                    ctx->stream));
-            HIP_TRY(hipMemcpyAsync(py.data(), ctx->S(1) + d.off_py, count * sizeof(float), hipMemcpyDeviceToHost,
+            HIP_TRY(hipMemcpyAsync(py.data(), ctx->S(1) + field_off(d, PSTD_FIELD_PY), count * sizeof(float), hipMemcpyDeviceToHost,
                                    ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             for (size_t i = 0; i < count; ++i) host[i] = host[i] + py[i];
