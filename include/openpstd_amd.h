@@ -259,6 +259,15 @@ int pstd_kernel_prepare(pstd_kernel *k, pstd_ctx **ctx);
  * The caller owns *ctx (pstd_ctx_destroy).  Domain d of scene i is domain
  * (sum of the earlier scenes' domain counts) + d; receivers likewise. */
 int pstd_batch_prepare(pstd_kernel *const *kernels, int n, int device, pstd_ctx **ctx);
+/* Dense derivative matrix of one segment's lines (n_out x K row-major, in
+ * double; n_out = n_main + 1 for pressure, n_main - 1 for velocity;
+ * K = n_main + 2*wlen), the operator the GEMM form multiplies: row i applied to
+ * [side1 tail (wlen) | main (n_main) | side2 head (wlen)] gives output i of
+ * spatderp3 (kernel/core/kernel_functions.cpp:123-318) for that line.
+ * coef = {c_side1, c_main1, c_side2, c_main2} (pstd_plan_segment).  Returns
+ * the element count (out may be NULL to query it) or a negative status. */
+long long pstd_derivative_matrix(int F, int table_len, int field, double dx, int wlen, int n_main, double patch_error,
+                                 const double coef[4], double *out, size_t cap);
 /* Plan the kernel hands to the device (segments etc.) — for tests. */
 int pstd_kernel_segment_count(pstd_kernel *k);
 int pstd_kernel_get_segment(pstd_kernel *k, int i, pstd_plan_segment *out);

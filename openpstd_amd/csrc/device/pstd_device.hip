@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "../../../include/openpstd_amd.h"
+#include "../host/plan_tables.h"
 #include "fft_line.h"
 
 using namespace pstdamd::dev;
@@ -600,32 +601,11 @@ struct pstd_ctx {
 
 namespace {
 
-// Derivative filter H[k] of a line of FFT length F (device plan cache).
-// derfact of WisdomCache::discretize_wave_numbers (kernel/core/WisdomCache.cpp:53-75)
-// for a table of length L: k_m = m * (pi/dx) / (L/2); pressure
-// exp(-i k dx/2) i k, velocity exp(+i k dx/2) i k.  spatderp3 multiplies bins
-// 0..F/2 (kernel/core/kernel_functions.cpp:234, 294) and FFTW's c2r ignores the
-// imaginary parts of bins 0 and F/2; extending Hermitian-symmetrically lets one
-// complex transform carry two real lines.  1/F (kernel_functions.cpp:243, 306)
-// is folded in.  Assembled in double.
+// Device plan cache entries: the tables of plan_tables.h rounded once to float.
 void build_filter(int F, int L, int field, double dx, std::vector<float2> &out) {
-    out.assign(F, make_float2(0, 0));
-    const double kmax = M_PI / dx;
-    const double dk = kmax / (L / 2);
-    std::vector<std::complex<double>> d(F / 2 + 1);
-    for (int m = 0; m <= F / 2; ++m) {
-        double k = m * dk;
-        std::complex<double> ik(0, k);
-        std::complex<double> sh = std::exp(std::complex<double>(0, (field == 0 ? -1 : 1) * k * dx * 0.5));
-        d[m] = sh * ik;
-    }
-    d[0] = 0;
-    d[F / 2] = std::complex<double>(d[F / 2].real(), 0);
-    for (int m = 0; m <= F / 2; ++m) {
-        std::complex<double> h = d[m] / (double)F;
-        out[m] = make_float2((float)h.real(), (float)h.imag());
-        if (m > 0 && m < F / 2) out[F - m] = make_float2((float)h.real(), (float)-h.imag());
-    }
+    const std::vector<std::complex<double>> h = pstdamd::derivative_filter(F, L, field, dx);
+    out.resize(F);
+    for (int m = 0; m < F; ++m) out[m] = make_float2((float)h[m].real(), (float)h[m].imag());
 }
 
 void build_twiddles(int F, std::vector<float2> &out) {
@@ -636,14 +616,9 @@ void build_twiddles(int F, std::vector<float2> &out) {
     }
 }
 
-// get_window_coefficients (kernel/core/kernel_functions.cpp:327-333) in double.
 void build_window(int w, int patch_error, std::vector<float> &out) {
-    out.resize(2 * w + 1);
-    double alpha = (patch_error - 40) / 20.0 + 1;
-    for (int i = 0; i <= 2 * w; ++i) {
-        double x = (double)(i - w) / w;
-        out[i] = (float)std::exp(-std::log(10.0) * alpha * std::pow(x, 6));
-    }
+    const std::vector<double> win = pstdamd::window_coefficients(w, patch_error);
+    out.assign(win.begin(), win.end());
 }
 
 // One RK stage.  sample_slot >= 0 (first stage only): the launch also records

@@ -4,6 +4,7 @@
 #include <string>
 #include <vector>
 
+#include "plan_tables.h"
 #include "pstd_kernel.h"
 
 using namespace OpenPSTD::Kernel;
@@ -250,6 +251,23 @@ int pstd_batch_prepare(pstd_kernel *const *kernels, int n, int device, pstd_ctx 
     }
     *out = ctx;
     return PSTD_OK;
+}
+
+long long pstd_derivative_matrix(int F, int table_len, int field, double dx, int wlen, int n_main, double patch_error,
+                                 const double coef[4], double *out, size_t cap) {
+    if (!coef || (field != 0 && field != 1)) return PSTD_ERR_INVALID;
+    pstdamd::SegmentShape s{F, table_len, field, dx, wlen, n_main, patch_error, coef[0], coef[1], coef[2], coef[3]};
+    if (!s.valid()) return PSTD_ERR_INVALID;
+    try {
+        const size_t n = (size_t)s.n_out() * s.K();
+        if (!out) return (long long)n;
+        if (cap < n) return PSTD_ERR_INVALID;
+        std::vector<double> D = pstdamd::derivative_matrix(s);
+        std::memcpy(out, D.data(), n * sizeof(double));
+        return (long long)n;
+    } catch (const std::exception &) {
+        return PSTD_ERR_INVALID;
+    }
 }
 
 int pstd_kernel_segment_count(pstd_kernel *k) {
