@@ -160,6 +160,16 @@ int pstd_ctx_create(int device, pstd_ctx **out);
 /* Uploads the scene plan: allocates device state (zeroed), builds the device
  * plan cache (derivative filters, windows, twiddles, line tiles). */
 int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan);
+/* Derivative form (pstd_ctx_set_form, before the plan is uploaded).
+ * PSTD_FORM_FFT: every line derivative as openPSTD computes it (windowed line,
+ *   FFT, spectral factor, inverse FFT) in FP32.
+ * PSTD_FORM_GEMM_F16: segments of at least 32 lines and 64 outputs apply the
+ *   segment's dense derivative matrix (pstd_derivative_matrix) on MFMA with
+ *   FP16 hi + lo split operands (inputs scaled per line by a power of two) and
+ *   FP32 accumulation; the other segments keep the FFT form. */
+#define PSTD_FORM_FFT 0
+#define PSTD_FORM_GEMM_F16 1
+int pstd_ctx_set_form(pstd_ctx *ctx, int form);
 /* Uploads several independent scenes as one plan: every RK stage of all of
  * them is one launch.  Domains and receivers are numbered across the scenes in
  * order (scene 0's domains first); segments name domains of their own scene.
@@ -254,10 +264,14 @@ int pstd_kernel_read_pml(pstd_kernel *k, int index, int which, float *out, size_
  * by the kernel; valid until the next prepare/run/destroy).  For callers that
  * drive the time loop themselves (benchmarks, stepwise inspection). */
 int pstd_kernel_prepare(pstd_kernel *k, pstd_ctx **ctx);
+/* Derivative form of the device contexts this kernel creates (PSTD_FORM_*,
+ * default PSTD_FORM_FFT); takes effect at the next run / prepare. */
+int pstd_kernel_set_form(pstd_kernel *k, int form);
 /* Builds one device context that advances the scenes of `n` configured
  * kernels together (pstd_ctx_upload_plans), initial fields set, frame 0 next.
  * The caller owns *ctx (pstd_ctx_destroy).  Domain d of scene i is domain
- * (sum of the earlier scenes' domain counts) + d; receivers likewise. */
+ * (sum of the earlier scenes' domain counts) + d; receivers likewise.  The
+ * derivative form is that of kernels[0]. */
 int pstd_batch_prepare(pstd_kernel *const *kernels, int n, int device, pstd_ctx **ctx);
 /* Dense derivative matrix of one segment's lines (n_out x K row-major, in
  * double; n_out = n_main + 1 for pressure, n_main - 1 for velocity;

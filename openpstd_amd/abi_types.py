@@ -17,6 +17,10 @@ FIELD_IDS = {"p0": 0, "px0": 1, "py0": 2, "vx0": 3, "vy0": 4}
 PSTD_RECEIVER_PHYSICAL = 0
 PSTD_RECEIVER_REFERENCE = 1
 
+PSTD_FORM_FFT = 0
+PSTD_FORM_GEMM_F16 = 1
+FORMS = {"fft": PSTD_FORM_FFT, "gemm": PSTD_FORM_GEMM_F16}
+
 
 class pstd_settings(C.Structure):
     _fields_ = [

@@ -29,6 +29,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <cstdio>
 #include <map>
 #include <string>
 #include <tuple>
@@ -516,6 +517,241 @@ __global__ void p0_materialise_kernel(const float *__restrict__ src, float *__re
     }
 }
 
+// ------------------------------------------------------------- GEMM form
+// The windowed line derivative of a segment is a fixed linear operator: the
+// dense matrix D (n_out x K) of plan_tables.h.  A block computes 128 lines x
+// 128 outputs of C = U D^T, U the raw input lines [side1 tail | main |
+// side2 head], on MFMA: each operand is split into FP16 hi + lo parts (U per
+// line scaled by a power of two so that its largest value is just below 2^15,
+// D by one power of two per matrix) and C = U_hi D_hi + U_hi D_lo + U_lo D_hi
+// accumulates in FP32 (v_mfma_f32_32x32x16_f16).  The epilogue undoes the
+// scaling and applies the RK update as the FFT form does.  X lines put U on
+// the MFMA A side (rows of C = lines); Y lines put D there (rows = outputs),
+// so the C columns that consecutive lanes hold are consecutive in memory in
+// both cases.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kGemmThreads = 256;
+constexpr int kGB = 128;        // lines and outputs per block
+constexpr int kGK = 32;         // k per LDS chunk (two MFMA k-steps)
+constexpr int kGKP = kGK + 8;   // LDS row pitch in halfs (80 B)
+
+struct GemmDesc {
+    TileDesc t;            // segment addressing; line0/count: this block's lines (count <= 128)
+    long long d_hi, d_lo;  // half offsets of D's parts in the matrix arena, row-major [n_pad][k_pad]
+    int k_pad, n0, d_exp, K;
+};
+
+struct GemmArgs {
+    StageArgs st;
+    const GemmDesc *__restrict__ tiles;
+    const _Float16 *dmat;
+};
+
+// Loads element (line, k) of the block's input lines for 16 (line, k) pairs of
+// one chunk: rows (dir 0) pass i covers lines 8i .. 8i+7 x 32 k (lanes along
+// k), columns (dir 1) covers 128 lines x 2 k (lanes along lines).
+__device__ __forceinline__ void gemm_load_chunk(const StageArgs &a, const TileDesc &td, int K, int k0, float (&val)[16]) {
+    const int tid = threadIdx.x, w = td.wlen, nm = td.n_main, vel = td.vel;
+    const float *A = vel ? a.vsrc : a.p0a;
+    const bool hasB = !vel && a.p0b != nullptr;
+    const float *B = hasB ? a.p0b : A;
+    const bool ov1 = k0 < w && td.s1_len >= 0, ovm = k0 < w + nm && k0 + kGK > w,
+               ov2 = k0 + kGK > w + nm && k0 < K && td.s2_len >= 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) val[i] = 0.f;
+    auto line_k = [&](int i, int &line, int &k) {
+        if (td.dir == 0) {
+            line = i * 8 + (tid >> 5);
+            k = k0 + (tid & 31);
+        } else {
+            line = tid & 127;
+            k = k0 + 2 * i + (tid >> 7);
+        }
+    };
+    if (ovm) {
+        const Rsrc rA = block_rsrc(A, td.m_dom, td.m_size);
+        const Rsrc rB = block_rsrc(B, td.m_dom + (td.m_b - td.m_a), td.m_size - (td.m_b - td.m_a));
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            int line, k;
+            line_k(i, line, k);
+            const bool ok = line < td.count && k >= w && k < w + nm;
+            const int off = ok ? (td.m_a + (td.line0 + line) * td.m_ls + (k - w) * td.m_ps) * 4 : kOOB;
+            val[i] += hasB ? bload(rA, off) + bload(rB, off) : bload(rA, off);
+        }
+    }
+    if (ov1) {
+        const Rsrc rA = block_rsrc(A, td.s1_dom, td.s1_size);
+        const Rsrc rB = block_rsrc(B, td.s1_dom + (td.s1_b - td.s1_a), td.s1_size - (td.s1_b - td.s1_a));
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            int line, k;
+            line_k(i, line, k);
+            const bool ok = line < td.count && k < w;
+            const int off =
+                ok ? (td.s1_a + (td.line0 + line + td.s1_line) * td.s1_ls + (td.s1_len - w - vel + k) * td.s1_ps) * 4
+                   : kOOB;
+            val[i] += hasB ? bload(rA, off) + bload(rB, off) : bload(rA, off);
+        }
+    }
+    if (ov2) {
+        const Rsrc rA = block_rsrc(A, td.s2_dom, td.s2_size);
+        const Rsrc rB = block_rsrc(B, td.s2_dom + (td.s2_b - td.s2_a), td.s2_size - (td.s2_b - td.s2_a));
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            int line, k;
+            line_k(i, line, k);
+            const bool ok = line < td.count && k >= w + nm && k < K;
+            const int off =
+                ok ? (td.s2_a + (td.line0 + line + td.s2_line) * td.s2_ls + (k - w - nm + vel) * td.s2_ps) * 4 : kOOB;
+            val[i] += hasB ? bload(rA, off) + bload(rB, off) : bload(rA, off);
+        }
+    }
+}
+
+__device__ __forceinline__ int gemm_line_of(const TileDesc &td, int i) {
+    return td.dir == 0 ? i * 8 + ((int)threadIdx.x >> 5) : ((int)threadIdx.x & 127);
+}
+__device__ __forceinline__ int gemm_k_of(const TileDesc &td, int i) {
+    return td.dir == 0 ? ((int)threadIdx.x & 31) : 2 * i + ((int)threadIdx.x >> 7);
+}
+
+__global__ void __launch_bounds__(kGemmThreads) stage_gemm_kernel(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) _Float16 uh[kGB * kGKP], ul[kGB * kGKP], dh[kGB * kGKP], dl[kGB * kGKP];
+    __shared__ int emax[kGB];
+    __shared__ float uscale[kGB];
+    __shared__ int uexp[kGB];
+    const StageArgs &a = g.st;
+    const GemmDesc gd = g.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
+    const TileDesc &td = gd.t;
+    const int tid = threadIdx.x, K = gd.K, nchunks = (K + kGK - 1) / kGK;
+    // 1. per-line scale: largest |u| of every input line of the block
+    if (tid < kGB) emax[tid] = 0;
+    __syncthreads();
+    {
+        float mx[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) mx[i] = 0.f;
+        for (int c = 0; c < nchunks; ++c) {
+            float v[16];
+            gemm_load_chunk(a, td, K, c * kGK, v);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) mx[i] = fmaxf(mx[i], fabsf(v[i]));
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) atomicMax(&emax[gemm_line_of(td, i)], __float_as_int(mx[i]));
+    }
+    __syncthreads();
+    if (tid < kGB) {
+        const float m = __int_as_float(emax[tid]);
+        const int e = m > 0.f ? min(max(14 - ilogbf(m), -100), 100) : 0;
+        uexp[tid] = e;
+        uscale[tid] = ldexpf(1.f, e);
+    }
+    // 2. K loop: chunk c+1 is loaded into registers while chunk c is multiplied
+    const _Float16 *Dh = g.dmat + gd.d_hi, *Dl = g.dmat + gd.d_lo;
+    const int drow = tid >> 1, dpart = (tid & 1) * 16;
+    float pre[16];
+    half8 pdh[2], pdl[2];
+    auto load_d = [&](int k0) {
+        const size_t o = (size_t)(gd.n0 + drow) * gd.k_pad + k0 + dpart;
+        pdh[0] = *reinterpret_cast<const half8 *>(Dh + o);
+        pdh[1] = *reinterpret_cast<const half8 *>(Dh + o + 8);
+        pdl[0] = *reinterpret_cast<const half8 *>(Dl + o);
+        pdl[1] = *reinterpret_cast<const half8 *>(Dl + o + 8);
+    };
+    gemm_load_chunk(a, td, K, 0, pre);
+    load_d(0);
+    const int wv = tid >> 6, lane = tid & 63, wr = wv >> 1, wc = wv & 1;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+    // rows of C come from R (U for X lines, D for Y lines), columns from Q
+    const _Float16 *Rh = td.dir == 0 ? uh : dh, *Rl = td.dir == 0 ? ul : dl;
+    const _Float16 *Qh = td.dir == 0 ? dh : uh, *Ql = td.dir == 0 ? dl : ul;
+    for (int c = 0; c < nchunks; ++c) {
+        __syncthreads();  // the previous chunk's MFMA reads are done
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int line = gemm_line_of(td, i), kk = gemm_k_of(td, i);
+            const float xs = pre[i] * uscale[line];
+            const _Float16 hi = (_Float16)xs;
+            uh[line * kGKP + kk] = hi;
+            ul[line * kGKP + kk] = (_Float16)(xs - (float)hi);
+        }
+        *reinterpret_cast<half8 *>(&dh[drow * kGKP + dpart]) = pdh[0];
+        *reinterpret_cast<half8 *>(&dh[drow * kGKP + dpart + 8]) = pdh[1];
+        *reinterpret_cast<half8 *>(&dl[drow * kGKP + dpart]) = pdl[0];
+        *reinterpret_cast<half8 *>(&dl[drow * kGKP + dpart + 8]) = pdl[1];
+        __syncthreads();
+        if (c + 1 < nchunks) {
+            gemm_load_chunk(a, td, K, (c + 1) * kGK, pre);
+            load_d((c + 1) * kGK);
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int kk = s * 16 + (lane >> 5) * 8;
+            half8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int row = wr * 64 + i * 32 + (lane & 31);
+                ah[i] = *reinterpret_cast<const half8 *>(&Rh[row * kGKP + kk]);
+                al[i] = *reinterpret_cast<const half8 *>(&Rl[row * kGKP + kk]);
+                const int col = wc * 64 + i * 32 + (lane & 31);
+                bh[i] = *reinterpret_cast<const half8 *>(&Qh[col * kGKP + kk]);
+                bl[i] = *reinterpret_cast<const half8 *>(&Ql[col * kGKP + kk]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                }
+        }
+    }
+    // 3. epilogue: L = C / (2^e_line 2^d_exp); out = t_n - s*coef*L (Solver::update_field_values)
+    const float sc = a.s * td.coef;
+    const bool use_pml = a.last && td.is_pml;
+    const Rsrc rt = block_rsrc(a.tn, td.m_dom, td.m_size);
+    const Rsrc rp = block_rsrc(a.pml, td.m_dom, td.m_size);
+    const Rsrc ro = block_rsrc(a.out, td.m_dom, td.m_size);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            int off[16];
+            float tnv[16], pm[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = wr * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                const int col = wc * 64 + j * 32 + (lane & 31);
+                const int line = td.dir == 0 ? row : col, n = gd.n0 + (td.dir == 0 ? col : row);
+                const bool ok = line < td.count && n < td.n_out;
+                off[r] = ok ? (td.o_f + (td.line0 + line) * td.o_ls + n * td.o_ps) * 4 : kOOB;
+                tnv[r] = bload(rt, off[r]);
+                pm[r] = use_pml ? bload(rp, off[r]) : 1.f;
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = wr * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                const int col = wc * 64 + j * 32 + (lane & 31);
+                const int line = td.dir == 0 ? row : col;
+                const float L = ldexpf(acc[i][j][r], -(uexp[line] + gd.d_exp));
+                const float nv = __builtin_fmaf(-sc, L, tnv[r]);
+                bstore(ro, off[r], nv);
+                if (a.last) bstore(rt, off[r], nv * pm[r]);
+            }
+        }
+}
+
 // ------------------------------------------------------------- host side
 
 #define HIP_TRY(expr)                                                                   \
@@ -544,6 +780,8 @@ struct pstd_ctx {
     std::vector<DomainDev> doms;
     std::vector<pstd_plan_domain> plan_domains;
     std::vector<TileDesc> tiles_small, tiles_4096, tiles_8192;
+    std::vector<GemmDesc> tiles_gemm;
+    int form = PSTD_FORM_FFT;
     size_t arena_floats = 0;
     float dt = 0, c1 = 0;
     float alpha[6] = {0};
@@ -554,6 +792,8 @@ struct pstd_ctx {
     float2 *ctab = nullptr;
     float *ftab = nullptr;
     TileDesc *d_tiles = nullptr;  // small | 4096 | 8192
+    GemmDesc *d_gemm = nullptr;
+    _Float16 *dmat = nullptr;     // derivative matrices of the GEMM form (hi, lo parts)
     ReceiverCell *d_cells = nullptr;
     P0Range *d_ranges = nullptr;
     int n_ranges = 0;
@@ -587,6 +827,10 @@ struct pstd_ctx {
         (void)hipFree(ctab);
         (void)hipFree(ftab);
         (void)hipFree(d_tiles);
+        (void)hipFree(d_gemm);
+        (void)hipFree(dmat);
+        d_gemm = nullptr;
+        dmat = nullptr;
         (void)hipFree(d_cells);
         (void)hipFree(d_ranges);
         (void)hipFree(d_samples);
@@ -670,6 +914,14 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
         size_t lds = 2 * sizeof(float2) * LdsMap<8192, 1, 0>::SIZE;
         if (lean) hipLaunchKernelGGL((stage_big_kernel<8192, true>), dim3(n8), dim3(1024), lds, ctx->stream, a);
         else hipLaunchKernelGGL((stage_big_kernel<8192, false>), dim3(n8), dim3(1024), lds, ctx->stream, a);
+    }
+    const int ng = (int)ctx->tiles_gemm.size();
+    if (ng) {
+        GemmArgs ga;
+        ga.st = a;
+        ga.tiles = ctx->d_gemm;
+        ga.dmat = ctx->dmat;
+        hipLaunchKernelGGL(stage_gemm_kernel, dim3(ng), dim3(kGemmThreads), 0, ctx->stream, ga);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -813,6 +1065,60 @@ static int autotune(pstd_ctx *ctx) {
     return PSTD_OK;
 }
 
+// GEMM form of one segment: its derivative matrix (built once per distinct
+// segment shape and stencil, split into FP16 hi + lo parts of D * 2^d_exp) and
+// its 128-line x 128-output blocks.
+static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_plan_segment &s, const TileDesc &g,
+                          std::vector<_Float16> &dmat, std::map<std::string, std::pair<long long, int>> &dcache) {
+    pstdamd::SegmentShape sh{g.F, s.table_len, s.field, (double)plan->dx, s.wlen, g.n_main, (double)plan->patch_error,
+                             (double)s.c_side1, (double)s.c_main1, (double)s.c_side2, (double)s.c_main2};
+    const int K = sh.K(), n_out = sh.n_out();
+    const int k_pad = (K + kGK - 1) / kGK * kGK, n_pad = (n_out + kGB - 1) / kGB * kGB;
+    char key[256];
+    std::snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%a/%a/%a", sh.F, sh.table_len, sh.field, sh.w, sh.n_main,
+                  sh.c_s1, sh.c_m1, sh.c_s2, sh.c_m2);
+    auto it = dcache.find(key);
+    if (it == dcache.end()) {
+        std::vector<double> D;
+        try {
+            D = pstdamd::derivative_matrix(sh);
+        } catch (const std::exception &e) {
+            ctx->err = std::string("GEMM form: ") + e.what();
+            return PSTD_ERR_INVALID;
+        }
+        double dm = 0;
+        for (double v : D) dm = std::max(dm, std::fabs(v));
+        const int d_exp = dm > 0 ? 14 - std::ilogb(dm) : 0;
+        const long long hi = (long long)dmat.size(), per = (long long)n_pad * k_pad;
+        dmat.resize(dmat.size() + 2 * per, (_Float16)0.f);
+        for (int i = 0; i < n_out; ++i)
+            for (int k = 0; k < K; ++k) {
+                const double v = std::ldexp(D[(size_t)i * K + k], d_exp);
+                const _Float16 h = (_Float16)v;
+                dmat[hi + (size_t)i * k_pad + k] = h;
+                dmat[hi + per + (size_t)i * k_pad + k] = (_Float16)(v - (double)h);
+            }
+        it = dcache.emplace(key, std::make_pair(hi, d_exp)).first;
+    }
+    const long long per = (long long)n_pad * k_pad;
+    for (int l = s.line_begin; l < s.line_end; l += kGB)
+        for (int n0 = 0; n0 < n_pad; n0 += kGB) {
+            GemmDesc gd{};
+            gd.t = g;
+            gd.t.kind = 3;
+            gd.t.line0 = l;
+            gd.t.count = std::min(kGB, s.line_end - l);
+            gd.d_hi = it->second.first;
+            gd.d_lo = it->second.first + per;
+            gd.k_pad = k_pad;
+            gd.n0 = n0;
+            gd.d_exp = it->second.second;
+            gd.K = K;
+            ctx->tiles_gemm.push_back(gd);
+        }
+    return PSTD_OK;
+}
+
 static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -821,6 +1127,7 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     ctx->tiles_small.clear();
     ctx->tiles_4096.clear();
     ctx->tiles_8192.clear();
+    ctx->tiles_gemm.clear();
     ctx->plan_domains.assign(plan->domains, plan->domains + plan->n_domains);
     // arena layout: one block per domain (vx, vy, px, py), 256 B aligned
     long long off = 0;
@@ -886,6 +1193,8 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         }
     };
     std::vector<std::vector<char>> covered(plan->n_domains, std::vector<char>(4, 0));
+    std::vector<_Float16> dmat;
+    std::map<std::string, std::pair<long long, int>> dcache;  // matrix key -> (hi offset, d_exp)
     for (int si = 0; si < plan->n_segments; ++si) {
         const pstd_plan_segment &s = plan->segments[si];
         if (s.domain < 0 || s.domain >= plan->n_domains || s.side1 >= plan->n_domains || s.side2 >= plan->n_domains ||
@@ -1000,6 +1309,13 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
             ftab.insert(ftab.end(), w.begin(), w.end());
         }
         g.win_off = wt->second;
+        // GEMM form: segments with enough lines and outputs become 128 x 128 GEMM blocks
+        const int nlines_seg = s.line_end - s.line_begin;
+        if (ctx->form == PSTD_FORM_GEMM_F16 && nlines_seg >= 32 && g.n_out >= 64) {
+            int rc = add_gemm_tiles(ctx, plan, s, g, dmat, dcache);
+            if (rc) return rc;
+            continue;
+        }
         // tiles: NL = 2 * P lines per block
         const int P = g.F >= 4096 ? 1 : kSmallThreads / (g.F / 8);
         const int per = 2 * P;
@@ -1091,6 +1407,14 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     if (!all.empty())
         HIP_TRY(hipMemcpyAsync(ctx->d_tiles, all.data(), all.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
                                ctx->stream));
+    if (!ctx->tiles_gemm.empty()) {
+        HIP_TRY(hipMalloc(&ctx->d_gemm, ctx->tiles_gemm.size() * sizeof(GemmDesc)));
+        HIP_TRY(hipMemcpyAsync(ctx->d_gemm, ctx->tiles_gemm.data(), ctx->tiles_gemm.size() * sizeof(GemmDesc),
+                               hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMalloc(&ctx->dmat, dmat.size() * sizeof(_Float16)));
+        HIP_TRY(hipMemcpyAsync(ctx->dmat, dmat.data(), dmat.size() * sizeof(_Float16), hipMemcpyHostToDevice,
+                               ctx->stream));
+    }
     HIP_TRY(hipMalloc(&ctx->d_cells, std::max<size_t>(1, cells.size()) * sizeof(ReceiverCell)));
     if (!cells.empty())
         HIP_TRY(hipMemcpyAsync(ctx->d_cells, cells.data(), cells.size() * sizeof(ReceiverCell), hipMemcpyHostToDevice,
@@ -1304,6 +1628,16 @@ int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps) {
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->drop_graph();
     ctx->graph_steps = steps;
+    return PSTD_OK;
+}
+
+int pstd_ctx_set_form(pstd_ctx *ctx, int form) {
+    if (!ctx || (form != PSTD_FORM_FFT && form != PSTD_FORM_GEMM_F16)) return PSTD_ERR_INVALID;
+    if (ctx->planned) {
+        ctx->err = "the derivative form is chosen before the plan is uploaded";
+        return PSTD_ERR_INVALID;
+    }
+    ctx->form = form;
     return PSTD_OK;
 }
 

@@ -216,6 +216,12 @@ int pstd_kernel_prepare(pstd_kernel *k, pstd_ctx **ctx) {
     });
 }
 
+int pstd_kernel_set_form(pstd_kernel *k, int form) {
+    if (!k || (form != PSTD_FORM_FFT && form != PSTD_FORM_GEMM_F16)) return PSTD_ERR_INVALID;
+    k->k->set_form(form);
+    return PSTD_OK;
+}
+
 int pstd_batch_prepare(pstd_kernel *const *kernels, int n, int device, pstd_ctx **out) {
     if (!kernels || n <= 0 || !out) return PSTD_ERR_INVALID;
     *out = nullptr;
@@ -231,7 +237,8 @@ int pstd_batch_prepare(pstd_kernel *const *kernels, int n, int device, pstd_ctx 
     pstd_ctx *ctx = nullptr;
     int rc = pstd_ctx_create(device, &ctx);
     if (rc) return rc;
-    rc = pstd_ctx_upload_plans(ctx, plans.data(), n);
+    rc = pstd_ctx_set_form(ctx, kernels[0]->k->get_form());
+    if (!rc) rc = pstd_ctx_upload_plans(ctx, plans.data(), n);
     int base = 0;
     for (int i = 0; i < n && rc == PSTD_OK; ++i) {
         const pstdamd::SceneModel &s = kernels[i]->k->get_scene();

@@ -64,8 +64,8 @@ std::shared_ptr<PSTDConfiguration> PSTDConfiguration::CreateEmptyConf() {
     return conf;
 }
 
-GPUSolver::GPUSolver(const pstdamd::SceneModel &scene, std::shared_ptr<KernelCallback> callback, int device)
-    : scene(scene), callback(std::move(callback)), device(device) {}
+GPUSolver::GPUSolver(const pstdamd::SceneModel &scene, std::shared_ptr<KernelCallback> callback, int device, int form)
+    : scene(scene), callback(std::move(callback)), device(device), form(form) {}
 
 GPUSolver::~GPUSolver() {
     if (ctx) pstd_ctx_destroy(ctx);
@@ -79,6 +79,7 @@ static void check_rc(pstd_ctx *ctx, int rc, const char *what) {
 void GPUSolver::prepare() {
     auto check = [&](int rc, const char *what) { check_rc(ctx, rc, what); };
     if (!ctx) check(pstd_ctx_create(device, &ctx), "pstd_ctx_create");
+    check(pstd_ctx_set_form(ctx, form), "pstd_ctx_set_form");
     pstd_plan_desc desc;
     std::vector<pstd_plan_domain> doms;
     std::vector<pstd_plan_receiver> rcv;
@@ -151,7 +152,7 @@ void PSTDKernel::run(std::shared_ptr<KernelCallback> callback) {
     if (!GPU)
         throw PSTDSolverError("this library provides the GPU solver only; construct PSTDKernel(GPU=true, ...)",
                               PSTD_ERR_UNSUPPORTED);
-    solver.reset(new GPUSolver(scene, callback));
+    solver.reset(new GPUSolver(scene, callback, 0, form));
     solver->compute_propagation();
 }
 
@@ -168,7 +169,7 @@ GPUSolver *PSTDKernel::prepare_solver() {
     if (!GPU)
         throw PSTDSolverError("this library provides the GPU solver only; construct PSTDKernel(GPU=true, ...)",
                               PSTD_ERR_UNSUPPORTED);
-    solver.reset(new GPUSolver(scene, std::make_shared<NullCallback>()));
+    solver.reset(new GPUSolver(scene, std::make_shared<NullCallback>(), 0, form));
     solver->prepare();
     return solver.get();
 }

@@ -16,7 +16,8 @@ namespace Kernel {
 // WriteSample per receiver when frame % SaveNth == 0.
 class GPUSolver {
 public:
-    GPUSolver(const pstdamd::SceneModel &scene, std::shared_ptr<KernelCallback> callback, int device = 0);
+    GPUSolver(const pstdamd::SceneModel &scene, std::shared_ptr<KernelCallback> callback, int device = 0,
+              int form = PSTD_FORM_FFT);
     ~GPUSolver();
     GPUSolver(const GPUSolver &) = delete;
     GPUSolver &operator=(const GPUSolver &) = delete;
@@ -30,6 +31,7 @@ private:
     const pstdamd::SceneModel &scene;
     std::shared_ptr<KernelCallback> callback;
     int device;
+    int form;
     pstd_ctx *ctx = nullptr;
 };
 
@@ -45,6 +47,9 @@ public:
     const pstdamd::SceneModel &get_scene() const { return scene; }
     bool configured() const { return (bool)config; }
     bool gpu() const { return GPU; }
+    // derivative form of the solvers created from now on (PSTD_FORM_*)
+    void set_form(int f) { form = f; }
+    int get_form() const { return form; }
     // solver of the last run (its device state stays readable until the next run)
     const GPUSolver *last_solver() const { return solver.get(); }
     // a solver with its device state initialised (frame 0 next), not run
@@ -52,6 +57,7 @@ public:
 
 private:
     bool GPU, MCPU;
+    int form = PSTD_FORM_FFT;
     std::shared_ptr<PSTDConfiguration> config;
     std::shared_ptr<KernelCallbackLog> callbackLog;
     pstdamd::SceneModel scene;

@@ -129,6 +129,11 @@ class PSTDKernel(KernelInterface):
         m.Framecount = fc.value
         return m
 
+    def set_form(self, form):
+        """Derivative form of the next run/prepare: "fft" (default) or "gemm"
+        (dense derivative matrices on MFMA, FP16 hi+lo split, FP32 accumulate)."""
+        _check(lib().pstd_kernel_set_form(self._h, T.FORMS[form]), self._err)
+
     def prepare(self):
         """Device context with the scene uploaded, frame 0 next (see DeviceSolver)."""
         return DeviceSolver(self)
@@ -273,12 +278,13 @@ class BatchSolver(_Context):
         b.field(scene, domain, "p0")
     """
 
-    def __init__(self, configs, device=0):
+    def __init__(self, configs, device=0, form="fft"):
         self.kernels = []
         for conf in configs:
             k = PSTDKernel(GPU=True)
             k.initialize_kernel(conf)
             self.kernels.append(k)
+        self.kernels[0].set_form(form)
         self.domain_base, self.receiver_base = [], []
         nd = nr = 0
         for k, conf in zip(self.kernels, configs):
