@@ -32,11 +32,13 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-RK-stage updates/sec (GEMM+batched vs plain FFT) at ≤1e-4 rel-L2 vs FP64"
 UNIT = "cell-RK-stage updates/s"
 HBM_PEAK_GBS = 8000.0
+MFMA_F16_PEAK_TFLOPS = 2500.0  # dense FP16 MFMA
 BYTES_PER_CELL_STAGE = 48
 
 
-WORKLOADS = ["lshape", "room", "square4096", "batch64"]
-DEFAULT_STEPS = {"lshape": 2000, "room": 500, "square4096": 100, "batch64": 100}
+WORKLOADS = ["lshape", "room", "square4096", "batch64", "sources256"]
+DEFAULT_STEPS = {"lshape": 2000, "room": 500, "square4096": 100, "batch64": 100, "sources256": 10}
+DEFAULT_WARMUP = {"lshape": 50, "room": 50, "square4096": 10, "batch64": 10, "sources256": 3}
 
 
 def workload(name, steps):
@@ -50,6 +52,9 @@ def workload(name, steps):
         return [W.room(steps)], "single 100x50-cell domain + 50-cell PML, centre source, 1 receiver, FP32 FFT path"
     if name == "square4096":
         return [W.square(0, steps)], "single 4096x4096-cell domain + 50-cell PML, 16 receivers, FP32 FFT path"
+    if name == "sources256":
+        return W.source_batch(0, steps), ("one 1024x1024-cell geometry (two 512x1024 domains + 50-cell PML) with 256 "
+                                          "seeded source positions advanced together, 8 receivers each")
     if name == "batch64":
         return W.scene_batch(0, steps), ("64 independent scenes (seed 0) of 1-4 rectangular domains, sides 64-1024 "
                                          "cells, edge absorption U[0,1], 1 source + 1 receiver each, advanced "
@@ -117,9 +122,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default depends on the workload)")
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default depends on the workload)")
     ap.add_argument("--workload", default="lshape", choices=WORKLOADS)
     ap.add_argument("--graph-steps", type=int, default=0, help="replay blocks of this many steps as a hipGraph")
+    ap.add_argument("--form", default="fft", choices=["fft", "gemm"],
+                    help="line derivative form: FP32 FFT, or dense matrices on MFMA (FP16 hi+lo split)")
     ap.add_argument("--tile-order", default="auto", choices=["auto", "xcd", "plain"],
                     help="workgroup-to-tile order (default: the autotuner's choice)")
     ap.add_argument("--flavour", default="auto", choices=["auto", "lean", "prefetch"],
@@ -139,14 +146,16 @@ def main():
     import openpstd_amd as P
 
     steps = args.steps if args.steps is not None else DEFAULT_STEPS[args.workload]
-    confs, desc = workload(args.workload, args.warmup + steps)
+    warmup = args.warmup if args.warmup is not None else DEFAULT_WARMUP[args.workload]
+    confs, desc = workload(args.workload, warmup + steps)
     if len(confs) == 1:
         kernel = P.PSTDKernel(GPU=True)
         kernel.initialize_kernel(confs[0])
+        kernel.set_form(args.form)
         kernels = [kernel]
         dev = kernel.prepare()
     else:
-        dev = P.BatchSolver(confs)
+        dev = P.BatchSolver(confs, form=args.form)
         kernels = dev.kernels
     dev.set_graph_steps(args.graph_steps)
     if args.tile_order != "auto":
@@ -156,7 +165,7 @@ def main():
     tuning = dev.tuning()
     cells = dev.cells_per_stage
 
-    dev.advance_async(args.warmup)
+    dev.advance_async(warmup)
     dev.synchronize()
     t0 = time.perf_counter()
     # HIP events on the solver stream around the timed steps; returns after they finished
@@ -167,6 +176,19 @@ def main():
     achieved = BYTES_PER_CELL_STAGE * cells / (mean_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.workload)
 
+    if args.form == "fft":
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                    "kernel": "stage kernel (one launch per RK stage)", "mean_launch_us": mean_ms * 1e3,
+                    "launches_timed": launches, "bytes_per_launch": BYTES_PER_CELL_STAGE * cells}
+    else:
+        flops, outs = dev.gemm_work()
+        tflops = flops / (mean_ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": tflops, "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": tflops / MFMA_F16_PEAK_TFLOPS, "traffic": None,
+                    "kernel": "GEMM-form stage launches (MFMA flops of the GEMM segments over the whole stage time)",
+                    "mean_launch_us": mean_ms * 1e3, "launches_timed": launches, "flops_per_stage": flops,
+                    "gemm_line_outputs_per_stage": outs}
     infos = [k.domain_info(i) for k in kernels for i in range(k.domain_count())]
     user = sum(d["nx"] * d["ny"] for d in infos if not d["is_pml"])
     out = {
@@ -175,22 +197,21 @@ def main():
         "unit": UNIT,
         "n_gpus": 1,
         "steps": steps,
-        "warmup": args.warmup,
+        "warmup": warmup,
         "ms_per_step": wall / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.form == "fft" else "f16x2-split/f32-accumulate",
         "data": "synthetic",
         "config": {"workload": desc, "scenes": len(confs), "cells_per_stage": cells, "user_cells": user,
                    "domains": len(infos), "rk_stages_per_step": 6, "graph_steps": args.graph_steps,
                    "launch_shape": {"lean": tuning["lean"], "xcd_order": tuning["xcd_order"],
                                     "autotuned": args.tile_order == "auto" and args.flavour == "auto"},
-                   "path": "FP32 FFT (fused LDS line derivative + RK update)"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stage kernel (one launch per RK stage)", "mean_launch_us": mean_ms * 1e3,
-                     "launches_timed": launches, "bytes_per_launch": BYTES_PER_CELL_STAGE * cells},
+                   "path": ("FP32 FFT (fused LDS line derivative + RK update)" if args.form == "fft" else
+                            "dense derivative matrices on MFMA, FP16 hi+lo split operands, FP32 accumulate "
+                            "(short segments in the FP32 FFT form)")},
+        "roofline": roofline,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(confs)

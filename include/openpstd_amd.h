@@ -217,6 +217,9 @@ int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps);
 int pstd_ctx_set_tile_order(pstd_ctx *ctx, int xcd_contiguous);
 int pstd_ctx_set_kernel_flavour(pstd_ctx *ctx, int lean);
 int pstd_ctx_get_tuning(const pstd_ctx *ctx, int32_t *lean, int32_t *xcd_contiguous, double *stage_ms);
+/* Work of the GEMM-form segments in one RK stage: algorithmic MFMA flops
+ * (3 split products x 2 x K x outputs x lines) and line outputs. */
+int pstd_ctx_gemm_work(const pstd_ctx *ctx, double *flops_per_stage, long long *outputs_per_stage);
 /* Cells updated per RK stage: sum of nx*ny over the non-rigid domains. */
 long long pstd_ctx_cells_per_stage(const pstd_ctx *ctx);
 /* Number of time steps advanced since upload. */

@@ -519,27 +519,34 @@ __global__ void p0_materialise_kernel(const float *__restrict__ src, float *__re
 
 // ------------------------------------------------------------- GEMM form
 // The windowed line derivative of a segment is a fixed linear operator: the
-// dense matrix D (n_out x K) of plan_tables.h.  A block computes 128 lines x
-// 128 outputs of C = U D^T, U the raw input lines [side1 tail | main |
-// side2 head], on MFMA: each operand is split into FP16 hi + lo parts (U per
-// line scaled by a power of two so that its largest value is just below 2^15,
-// D by one power of two per matrix) and C = U_hi D_hi + U_hi D_lo + U_lo D_hi
-// accumulates in FP32 (v_mfma_f32_32x32x16_f16).  The epilogue undoes the
-// scaling and applies the RK update as the FFT form does.  X lines put U on
-// the MFMA A side (rows of C = lines); Y lines put D there (rows = outputs),
-// so the C columns that consecutive lanes hold are consecutive in memory in
-// both cases.
+// dense matrix D (n_out x K) of plan_tables.h.  Per RK stage, for every block
+// of up to 128 lines of a GEMM-form segment:
+//  1. stage_split_kernel gathers the raw input lines U = [side1 tail | main |
+//     side2 head] (K values each), scales every line by the power of two that
+//     puts its largest value just below 2^15, and stores U as FP16 hi + lo
+//     parts (U = hi + lo to 22 bits) in k-contiguous rows;
+//  2. stage_gemm_kernel computes 128 lines x 128 outputs of C = U D^T on MFMA
+//     (v_mfma_f32_32x32x16_f16) as U_hi D_hi + U_hi D_lo + U_lo D_hi with FP32
+//     accumulation (D split the same way on the host, one power of two per
+//     matrix), undoes both scalings and applies the RK update as the FFT form
+//     does.  X lines put U on the MFMA A side (rows of C = lines); Y lines put
+//     D there (rows = outputs), so that the C columns consecutive lanes hold
+//     are consecutive in memory in both directions.
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kGemmThreads = 256;
-constexpr int kGB = 128;        // lines and outputs per block
-constexpr int kGK = 32;         // k per LDS chunk (two MFMA k-steps)
-constexpr int kGKP = kGK + 8;   // LDS row pitch in halfs (80 B)
+constexpr int kGB = 128;        // lines per line block, outputs per GEMM block
+constexpr int kCK = 32;         // k per chunk of the split pass
+constexpr int kGK = 64;         // k per LDS chunk of the GEMM (four MFMA k-steps)
+constexpr int kGKP = kGK + 8;   // GEMM LDS row pitch in halfs (144 B)
+constexpr int kCKP = kCK + 8;   // split-pass LDS row pitch in halfs (80 B)
 
 struct GemmDesc {
-    TileDesc t;            // segment addressing; line0/count: this block's lines (count <= 128)
-    long long d_hi, d_lo;  // half offsets of D's parts in the matrix arena, row-major [n_pad][k_pad]
+    TileDesc t;            // segment addressing; line0/count: the line block (count <= 128)
+    long long d_hi, d_lo;  // D parts (halfs), row-major [n_pad][k_pad]
+    long long u_hi, u_lo;  // the line block's split inputs (halfs), row-major [128][k_pad]
+    long long e_off;       // the line block's 128 scale exponents
     int k_pad, n0, d_exp, K;
 };
 
@@ -547,36 +554,36 @@ struct GemmArgs {
     StageArgs st;
     const GemmDesc *__restrict__ tiles;
     const _Float16 *dmat;
+    _Float16 *umat;
+    int *uexp;
 };
 
-// Loads element (line, k) of the block's input lines for 16 (line, k) pairs of
-// one chunk: rows (dir 0) pass i covers lines 8i .. 8i+7 x 32 k (lanes along
-// k), columns (dir 1) covers 128 lines x 2 k (lanes along lines).
-__device__ __forceinline__ void gemm_load_chunk(const StageArgs &a, const TileDesc &td, int K, int k0, float (&val)[16]) {
-    const int tid = threadIdx.x, w = td.wlen, nm = td.n_main, vel = td.vel;
+// Element (line, k) of the line block for 16 (line, k) pairs of one chunk of
+// kCK k: rows (dir 0) pass i covers lines 8i .. 8i+7 with lanes along k,
+// columns (dir 1) covers 128 lines x 2 k with lanes along the lines.
+__device__ __forceinline__ int split_line(const TileDesc &td, int i) {
+    return td.dir == 0 ? i * 8 + ((int)threadIdx.x >> 5) : ((int)threadIdx.x & 127);
+}
+__device__ __forceinline__ int split_k(const TileDesc &td, int i) {
+    return td.dir == 0 ? ((int)threadIdx.x & 31) : 2 * i + ((int)threadIdx.x >> 7);
+}
+
+__device__ __forceinline__ void split_load_chunk(const StageArgs &a, const TileDesc &td, int K, int k0,
+                                                 float (&val)[16]) {
+    const int w = td.wlen, nm = td.n_main, vel = td.vel;
     const float *A = vel ? a.vsrc : a.p0a;
     const bool hasB = !vel && a.p0b != nullptr;
     const float *B = hasB ? a.p0b : A;
-    const bool ov1 = k0 < w && td.s1_len >= 0, ovm = k0 < w + nm && k0 + kGK > w,
-               ov2 = k0 + kGK > w + nm && k0 < K && td.s2_len >= 0;
+    const bool ov1 = k0 < w && td.s1_len >= 0, ovm = k0 < w + nm && k0 + kCK > w,
+               ov2 = k0 + kCK > w + nm && k0 < K && td.s2_len >= 0;
 #pragma unroll
     for (int i = 0; i < 16; ++i) val[i] = 0.f;
-    auto line_k = [&](int i, int &line, int &k) {
-        if (td.dir == 0) {
-            line = i * 8 + (tid >> 5);
-            k = k0 + (tid & 31);
-        } else {
-            line = tid & 127;
-            k = k0 + 2 * i + (tid >> 7);
-        }
-    };
     if (ovm) {
         const Rsrc rA = block_rsrc(A, td.m_dom, td.m_size);
         const Rsrc rB = block_rsrc(B, td.m_dom + (td.m_b - td.m_a), td.m_size - (td.m_b - td.m_a));
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            int line, k;
-            line_k(i, line, k);
+            const int line = split_line(td, i), k = k0 + split_k(td, i);
             const bool ok = line < td.count && k >= w && k < w + nm;
             const int off = ok ? (td.m_a + (td.line0 + line) * td.m_ls + (k - w) * td.m_ps) * 4 : kOOB;
             val[i] += hasB ? bload(rA, off) + bload(rB, off) : bload(rA, off);
@@ -587,8 +594,7 @@ __device__ __forceinline__ void gemm_load_chunk(const StageArgs &a, const TileDe
         const Rsrc rB = block_rsrc(B, td.s1_dom + (td.s1_b - td.s1_a), td.s1_size - (td.s1_b - td.s1_a));
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            int line, k;
-            line_k(i, line, k);
+            const int line = split_line(td, i), k = k0 + split_k(td, i);
             const bool ok = line < td.count && k < w;
             const int off =
                 ok ? (td.s1_a + (td.line0 + line + td.s1_line) * td.s1_ls + (td.s1_len - w - vel + k) * td.s1_ps) * 4
@@ -601,8 +607,7 @@ __device__ __forceinline__ void gemm_load_chunk(const StageArgs &a, const TileDe
         const Rsrc rB = block_rsrc(B, td.s2_dom + (td.s2_b - td.s2_a), td.s2_size - (td.s2_b - td.s2_a));
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            int line, k;
-            line_k(i, line, k);
+            const int line = split_line(td, i), k = k0 + split_k(td, i);
             const bool ok = line < td.count && k >= w + nm && k < K;
             const int off =
                 ok ? (td.s2_a + (td.line0 + line + td.s2_line) * td.s2_ls + (k - w - nm + vel) * td.s2_ps) * 4 : kOOB;
@@ -611,59 +616,88 @@ __device__ __forceinline__ void gemm_load_chunk(const StageArgs &a, const TileDe
     }
 }
 
-__device__ __forceinline__ int gemm_line_of(const TileDesc &td, int i) {
-    return td.dir == 0 ? i * 8 + ((int)threadIdx.x >> 5) : ((int)threadIdx.x & 127);
-}
-__device__ __forceinline__ int gemm_k_of(const TileDesc &td, int i) {
-    return td.dir == 0 ? ((int)threadIdx.x & 31) : 2 * i + ((int)threadIdx.x >> 7);
-}
-
-__global__ void __launch_bounds__(kGemmThreads) stage_gemm_kernel(GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) _Float16 uh[kGB * kGKP], ul[kGB * kGKP], dh[kGB * kGKP], dl[kGB * kGKP];
+__global__ void __launch_bounds__(kGemmThreads) stage_split_kernel(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) _Float16 th[kGB * kCKP], tl[kGB * kCKP];
     __shared__ int emax[kGB];
-    __shared__ float uscale[kGB];
-    __shared__ int uexp[kGB];
+    __shared__ float escale[kGB];
     const StageArgs &a = g.st;
-    const GemmDesc gd = g.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
+    const GemmDesc gd = g.tiles[blockIdx.x];
     const TileDesc &td = gd.t;
-    const int tid = threadIdx.x, K = gd.K, nchunks = (K + kGK - 1) / kGK;
-    // 1. per-line scale: largest |u| of every input line of the block
+    const int tid = threadIdx.x, K = gd.K, nch = gd.k_pad / kCK;
     if (tid < kGB) emax[tid] = 0;
     __syncthreads();
+    // largest |u| of every line (two chunks of loads in flight)
     {
         float mx[16];
 #pragma unroll
         for (int i = 0; i < 16; ++i) mx[i] = 0.f;
-        for (int c = 0; c < nchunks; ++c) {
-            float v[16];
-            gemm_load_chunk(a, td, K, c * kGK, v);
+        for (int c = 0; c < nch; c += 2) {
+            float v0[16], v1[16];
+            split_load_chunk(a, td, K, c * kCK, v0);
+            split_load_chunk(a, td, K, (c + 1) * kCK, v1);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) mx[i] = fmaxf(mx[i], fabsf(v[i]));
+            for (int i = 0; i < 16; ++i) mx[i] = fmaxf(mx[i], fmaxf(fabsf(v0[i]), fabsf(v1[i])));
         }
 #pragma unroll
-        for (int i = 0; i < 16; ++i) atomicMax(&emax[gemm_line_of(td, i)], __float_as_int(mx[i]));
+        for (int i = 0; i < 16; ++i) atomicMax(&emax[split_line(td, i)], __float_as_int(mx[i]));
     }
     __syncthreads();
     if (tid < kGB) {
         const float m = __int_as_float(emax[tid]);
         const int e = m > 0.f ? min(max(14 - ilogbf(m), -100), 100) : 0;
-        uexp[tid] = e;
-        uscale[tid] = ldexpf(1.f, e);
+        escale[tid] = ldexpf(1.f, e);
+        g.uexp[gd.e_off + tid] = e;
     }
-    // 2. K loop: chunk c+1 is loaded into registers while chunk c is multiplied
-    const _Float16 *Dh = g.dmat + gd.d_hi, *Dl = g.dmat + gd.d_lo;
-    const int drow = tid >> 1, dpart = (tid & 1) * 16;
-    float pre[16];
-    half8 pdh[2], pdl[2];
-    auto load_d = [&](int k0) {
-        const size_t o = (size_t)(gd.n0 + drow) * gd.k_pad + k0 + dpart;
-        pdh[0] = *reinterpret_cast<const half8 *>(Dh + o);
-        pdh[1] = *reinterpret_cast<const half8 *>(Dh + o + 8);
-        pdl[0] = *reinterpret_cast<const half8 *>(Dl + o);
-        pdl[1] = *reinterpret_cast<const half8 *>(Dl + o + 8);
+    __syncthreads();
+    _Float16 *Uh = g.umat + gd.u_hi, *Ul = g.umat + gd.u_lo;
+    for (int c = 0; c < nch; ++c) {
+        float v[16];
+        split_load_chunk(a, td, K, c * kCK, v);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int line = split_line(td, i), kk = split_k(td, i);
+            const float xs = v[i] * escale[line];
+            const _Float16 hi = (_Float16)xs;
+            th[line * kCKP + kk] = hi;
+            tl[line * kCKP + kk] = (_Float16)(xs - (float)hi);
+        }
+        __syncthreads();
+        // 128 rows x 32 halfs per part: 512 vectors of 8 halfs; two per thread and part
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int vi = tid + j * kGemmThreads, row = vi >> 2, col = (vi & 3) * 8;
+            const size_t o = (size_t)row * gd.k_pad + c * kCK + col;
+            *reinterpret_cast<half8 *>(Uh + o) = *reinterpret_cast<const half8 *>(&th[row * kCKP + col]);
+            *reinterpret_cast<half8 *>(Ul + o) = *reinterpret_cast<const half8 *>(&tl[row * kCKP + col]);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kGemmThreads) stage_gemm_kernel(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) _Float16 uh[kGB * kGKP], ul[kGB * kGKP], dh[kGB * kGKP], dl[kGB * kGKP];
+    __shared__ int uexp[kGB];
+    const StageArgs &a = g.st;
+    const GemmDesc gd = g.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
+    const TileDesc &td = gd.t;
+    const int tid = threadIdx.x, nch = gd.k_pad / kGK;
+    if (tid < kGB) uexp[tid] = g.uexp[gd.e_off + tid];
+    // operand chunks: 128 rows x 64 halfs per part = 1024 vectors; four per thread and part
+    const _Float16 *Uh = g.umat + gd.u_hi, *Ul = g.umat + gd.u_lo;
+    const _Float16 *Dh = g.dmat + gd.d_hi + (size_t)gd.n0 * gd.k_pad, *Dl = g.dmat + gd.d_lo + (size_t)gd.n0 * gd.k_pad;
+    half8 pu_h[4], pu_l[4], pd_h[4], pd_l[4];
+    auto load_chunk = [&](int c) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int vi = tid + j * kGemmThreads, row = vi >> 3, col = (vi & 7) * 8;
+            const size_t o = (size_t)row * gd.k_pad + c * kGK + col;
+            pu_h[j] = *reinterpret_cast<const half8 *>(Uh + o);
+            pu_l[j] = *reinterpret_cast<const half8 *>(Ul + o);
+            pd_h[j] = *reinterpret_cast<const half8 *>(Dh + o);
+            pd_l[j] = *reinterpret_cast<const half8 *>(Dl + o);
+        }
     };
-    gemm_load_chunk(a, td, K, 0, pre);
-    load_d(0);
+    load_chunk(0);
     const int wv = tid >> 6, lane = tid & 63, wr = wv >> 1, wc = wv & 1;
     f32x16 acc[2][2];
 #pragma unroll
@@ -675,27 +709,20 @@ __global__ void __launch_bounds__(kGemmThreads) stage_gemm_kernel(GemmArgs g) {
     // rows of C come from R (U for X lines, D for Y lines), columns from Q
     const _Float16 *Rh = td.dir == 0 ? uh : dh, *Rl = td.dir == 0 ? ul : dl;
     const _Float16 *Qh = td.dir == 0 ? dh : uh, *Ql = td.dir == 0 ? dl : ul;
-    for (int c = 0; c < nchunks; ++c) {
+    for (int c = 0; c < nch; ++c) {
         __syncthreads();  // the previous chunk's MFMA reads are done
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int line = gemm_line_of(td, i), kk = gemm_k_of(td, i);
-            const float xs = pre[i] * uscale[line];
-            const _Float16 hi = (_Float16)xs;
-            uh[line * kGKP + kk] = hi;
-            ul[line * kGKP + kk] = (_Float16)(xs - (float)hi);
+        for (int j = 0; j < 4; ++j) {
+            const int vi = tid + j * kGemmThreads, row = vi >> 3, col = (vi & 7) * 8;
+            *reinterpret_cast<half8 *>(&uh[row * kGKP + col]) = pu_h[j];
+            *reinterpret_cast<half8 *>(&ul[row * kGKP + col]) = pu_l[j];
+            *reinterpret_cast<half8 *>(&dh[row * kGKP + col]) = pd_h[j];
+            *reinterpret_cast<half8 *>(&dl[row * kGKP + col]) = pd_l[j];
         }
-        *reinterpret_cast<half8 *>(&dh[drow * kGKP + dpart]) = pdh[0];
-        *reinterpret_cast<half8 *>(&dh[drow * kGKP + dpart + 8]) = pdh[1];
-        *reinterpret_cast<half8 *>(&dl[drow * kGKP + dpart]) = pdl[0];
-        *reinterpret_cast<half8 *>(&dl[drow * kGKP + dpart + 8]) = pdl[1];
         __syncthreads();
-        if (c + 1 < nchunks) {
-            gemm_load_chunk(a, td, K, (c + 1) * kGK, pre);
-            load_d((c + 1) * kGK);
-        }
+        if (c + 1 < nch) load_chunk(c + 1);
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < kGK / 16; ++s) {
             const int kk = s * 16 + (lane >> 5) * 8;
             half8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
@@ -717,7 +744,7 @@ __global__ void __launch_bounds__(kGemmThreads) stage_gemm_kernel(GemmArgs g) {
                 }
         }
     }
-    // 3. epilogue: L = C / (2^e_line 2^d_exp); out = t_n - s*coef*L (Solver::update_field_values)
+    // epilogue: L = C / (2^e_line 2^d_exp); out = t_n - s*coef*L (Solver::update_field_values)
     const float sc = a.s * td.coef;
     const bool use_pml = a.last && td.is_pml;
     const Rsrc rt = block_rsrc(a.tn, td.m_dom, td.m_size);
@@ -780,8 +807,11 @@ struct pstd_ctx {
     std::vector<DomainDev> doms;
     std::vector<pstd_plan_domain> plan_domains;
     std::vector<TileDesc> tiles_small, tiles_4096, tiles_8192;
-    std::vector<GemmDesc> tiles_gemm;
+    std::vector<GemmDesc> tiles_gemm, tiles_split;
+    long long u_halfs = 0, u_exps = 0;  // split-input scratch of the GEMM form
     int form = PSTD_FORM_FFT;
+    double gemm_flops = 0;   // algorithmic MFMA flops of one stage (3 split products)
+    long long gemm_cells = 0;
     size_t arena_floats = 0;
     float dt = 0, c1 = 0;
     float alpha[6] = {0};
@@ -792,8 +822,10 @@ struct pstd_ctx {
     float2 *ctab = nullptr;
     float *ftab = nullptr;
     TileDesc *d_tiles = nullptr;  // small | 4096 | 8192
-    GemmDesc *d_gemm = nullptr;
+    GemmDesc *d_gemm = nullptr, *d_split = nullptr;
     _Float16 *dmat = nullptr;     // derivative matrices of the GEMM form (hi, lo parts)
+    _Float16 *umat = nullptr;     // split input lines of the GEMM form (rewritten every stage)
+    int *uexp = nullptr;
     ReceiverCell *d_cells = nullptr;
     P0Range *d_ranges = nullptr;
     int n_ranges = 0;
@@ -828,9 +860,13 @@ struct pstd_ctx {
         (void)hipFree(ftab);
         (void)hipFree(d_tiles);
         (void)hipFree(d_gemm);
+        (void)hipFree(d_split);
         (void)hipFree(dmat);
-        d_gemm = nullptr;
-        dmat = nullptr;
+        (void)hipFree(umat);
+        (void)hipFree(uexp);
+        d_gemm = d_split = nullptr;
+        dmat = umat = nullptr;
+        uexp = nullptr;
         (void)hipFree(d_cells);
         (void)hipFree(d_ranges);
         (void)hipFree(d_samples);
@@ -915,12 +951,16 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
         if (lean) hipLaunchKernelGGL((stage_big_kernel<8192, true>), dim3(n8), dim3(1024), lds, ctx->stream, a);
         else hipLaunchKernelGGL((stage_big_kernel<8192, false>), dim3(n8), dim3(1024), lds, ctx->stream, a);
     }
-    const int ng = (int)ctx->tiles_gemm.size();
+    const int ng = (int)ctx->tiles_gemm.size(), nsp = (int)ctx->tiles_split.size();
     if (ng) {
         GemmArgs ga;
         ga.st = a;
-        ga.tiles = ctx->d_gemm;
         ga.dmat = ctx->dmat;
+        ga.umat = ctx->umat;
+        ga.uexp = ctx->uexp;
+        ga.tiles = ctx->d_split;
+        hipLaunchKernelGGL(stage_split_kernel, dim3(nsp), dim3(kGemmThreads), 0, ctx->stream, ga);
+        ga.tiles = ctx->d_gemm;
         hipLaunchKernelGGL(stage_gemm_kernel, dim3(ng), dim3(kGemmThreads), 0, ctx->stream, ga);
     }
     hipError_t e = hipGetLastError();
@@ -1073,7 +1113,7 @@ static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_
     pstdamd::SegmentShape sh{g.F, s.table_len, s.field, (double)plan->dx, s.wlen, g.n_main, (double)plan->patch_error,
                              (double)s.c_side1, (double)s.c_main1, (double)s.c_side2, (double)s.c_main2};
     const int K = sh.K(), n_out = sh.n_out();
-    const int k_pad = (K + kGK - 1) / kGK * kGK, n_pad = (n_out + kGB - 1) / kGB * kGB;
+    const int k_pad = (K + kGK - 1) / kGK * kGK, n_pad = (n_out + kGB - 1) / kGB * kGB;  // kGK is a multiple of kCK
     char key[256];
     std::snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%a/%a/%a", sh.F, sh.table_len, sh.field, sh.w, sh.n_main,
                   sh.c_s1, sh.c_m1, sh.c_s2, sh.c_m2);
@@ -1101,21 +1141,30 @@ static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_
         it = dcache.emplace(key, std::make_pair(hi, d_exp)).first;
     }
     const long long per = (long long)n_pad * k_pad;
-    for (int l = s.line_begin; l < s.line_end; l += kGB)
+    ctx->gemm_flops += 3.0 * 2.0 * K * (double)n_out * (s.line_end - s.line_begin);
+    ctx->gemm_cells += (long long)n_out * (s.line_end - s.line_begin);
+    for (int l = s.line_begin; l < s.line_end; l += kGB) {
+        GemmDesc gd{};
+        gd.t = g;
+        gd.t.kind = 3;
+        gd.t.line0 = l;
+        gd.t.count = std::min(kGB, s.line_end - l);
+        gd.d_hi = it->second.first;
+        gd.d_lo = it->second.first + per;
+        gd.u_hi = ctx->u_halfs;
+        gd.u_lo = ctx->u_halfs + (long long)kGB * k_pad;
+        gd.e_off = ctx->u_exps;
+        ctx->u_halfs += 2LL * kGB * k_pad;
+        ctx->u_exps += kGB;
+        gd.k_pad = k_pad;
+        gd.d_exp = it->second.second;
+        gd.K = K;
+        ctx->tiles_split.push_back(gd);
         for (int n0 = 0; n0 < n_pad; n0 += kGB) {
-            GemmDesc gd{};
-            gd.t = g;
-            gd.t.kind = 3;
-            gd.t.line0 = l;
-            gd.t.count = std::min(kGB, s.line_end - l);
-            gd.d_hi = it->second.first;
-            gd.d_lo = it->second.first + per;
-            gd.k_pad = k_pad;
             gd.n0 = n0;
-            gd.d_exp = it->second.second;
-            gd.K = K;
             ctx->tiles_gemm.push_back(gd);
         }
+    }
     return PSTD_OK;
 }
 
@@ -1128,6 +1177,10 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     ctx->tiles_4096.clear();
     ctx->tiles_8192.clear();
     ctx->tiles_gemm.clear();
+    ctx->tiles_split.clear();
+    ctx->u_halfs = ctx->u_exps = 0;
+    ctx->gemm_flops = 0;
+    ctx->gemm_cells = 0;
     ctx->plan_domains.assign(plan->domains, plan->domains + plan->n_domains);
     // arena layout: one block per domain (vx, vy, px, py), 256 B aligned
     long long off = 0;
@@ -1411,9 +1464,14 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         HIP_TRY(hipMalloc(&ctx->d_gemm, ctx->tiles_gemm.size() * sizeof(GemmDesc)));
         HIP_TRY(hipMemcpyAsync(ctx->d_gemm, ctx->tiles_gemm.data(), ctx->tiles_gemm.size() * sizeof(GemmDesc),
                                hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMalloc(&ctx->d_split, ctx->tiles_split.size() * sizeof(GemmDesc)));
+        HIP_TRY(hipMemcpyAsync(ctx->d_split, ctx->tiles_split.data(), ctx->tiles_split.size() * sizeof(GemmDesc),
+                               hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(hipMalloc(&ctx->dmat, dmat.size() * sizeof(_Float16)));
         HIP_TRY(hipMemcpyAsync(ctx->dmat, dmat.data(), dmat.size() * sizeof(_Float16), hipMemcpyHostToDevice,
                                ctx->stream));
+        HIP_TRY(hipMalloc(&ctx->umat, (size_t)ctx->u_halfs * sizeof(_Float16)));
+        HIP_TRY(hipMalloc(&ctx->uexp, (size_t)ctx->u_exps * sizeof(int)));
     }
     HIP_TRY(hipMalloc(&ctx->d_cells, std::max<size_t>(1, cells.size()) * sizeof(ReceiverCell)));
     if (!cells.empty())
@@ -1685,6 +1743,13 @@ int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int
     if (rc) return rc;
     *mean_stage_ms = (double)ms / (6.0 * steps);
     if (launches) *launches = 6 * steps;
+    return PSTD_OK;
+}
+
+int pstd_ctx_gemm_work(const pstd_ctx *ctx, double *flops_per_stage, long long *outputs_per_stage) {
+    if (!ctx || !ctx->planned) return PSTD_ERR_INVALID;
+    if (flops_per_stage) *flops_per_stage = ctx->gemm_flops;
+    if (outputs_per_stage) *outputs_per_stage = ctx->gemm_cells;
     return PSTD_OK;
 }
 

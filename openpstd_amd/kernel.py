@@ -219,6 +219,12 @@ class _Context:
         _check(lib().pstd_ctx_profile_stages(self.ctx, steps, C.byref(ms), C.byref(n)), self._err)
         return ms.value, n.value
 
+    def gemm_work(self):
+        """(MFMA flops, line outputs) of the GEMM-form segments per RK stage."""
+        f, n = C.c_double(), C.c_longlong()
+        _check(lib().pstd_ctx_gemm_work(self.ctx, C.byref(f), C.byref(n)), self._err)
+        return f.value, n.value
+
     def set_graph_steps(self, steps):
         """Replay blocks of `steps` time steps as one captured hipGraph (0: off)."""
         _check(lib().pstd_ctx_set_graph_steps(self.ctx, steps), self._err)
