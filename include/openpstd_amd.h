@@ -163,10 +163,11 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan);
 /* Derivative form (pstd_ctx_set_form, before the plan is uploaded).
  * PSTD_FORM_FFT: every line derivative as openPSTD computes it (windowed line,
  *   FFT, spectral factor, inverse FFT) in FP32.
- * PSTD_FORM_GEMM_F16: segments of at least 32 lines and 64 outputs apply the
- *   segment's dense derivative matrix (pstd_derivative_matrix) on MFMA with
- *   FP16 hi + lo split operands (inputs scaled per line by a power of two) and
- *   FP32 accumulation; the other segments keep the FFT form. */
+ * PSTD_FORM_GEMM_F16: segments whose lines and outputs fill 128 x 128 blocks
+ *   at least to 3/4 apply the segment's dense derivative matrix
+ *   (pstd_derivative_matrix) on MFMA with FP16 hi + lo split operands (inputs
+ *   scaled per line by a power of two) and FP32 accumulation; the other
+ *   segments (thin PML strips) keep the FFT form. */
 #define PSTD_FORM_FFT 0
 #define PSTD_FORM_GEMM_F16 1
 int pstd_ctx_set_form(pstd_ctx *ctx, int form);

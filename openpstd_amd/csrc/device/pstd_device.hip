@@ -521,10 +521,11 @@ __global__ void p0_materialise_kernel(const float *__restrict__ src, float *__re
 // The windowed line derivative of a segment is a fixed linear operator: the
 // dense matrix D (n_out x K) of plan_tables.h.  Per RK stage, for every block
 // of up to 128 lines of a GEMM-form segment:
-//  1. stage_split_kernel gathers the raw input lines U = [side1 tail | main |
-//     side2 head] (K values each), scales every line by the power of two that
-//     puts its largest value just below 2^15, and stores U as FP16 hi + lo
-//     parts (U = hi + lo to 22 bits) in k-contiguous rows;
+//  1. stage_umax_kernel and stage_usplit_kernel gather the raw input lines
+//     U = [side1 tail | main | side2 head] (K values each), scale every line
+//     by the power of two that puts its largest value just below 2^15, and
+//     store U as FP16 hi + lo parts (U = hi + lo to 22 bits) in k-contiguous
+//     rows;
 //  2. stage_gemm_kernel computes 128 lines x 128 outputs of C = U D^T on MFMA
 //     (v_mfma_f32_32x32x16_f16) as U_hi D_hi + U_hi D_lo + U_lo D_hi with FP32
 //     accumulation (D split the same way on the host, one power of two per
@@ -540,7 +541,6 @@ constexpr int kGB = 128;        // lines per line block, outputs per GEMM block
 constexpr int kCK = 32;         // k per chunk of the split pass
 constexpr int kGK = 64;         // k per LDS chunk of the GEMM (four MFMA k-steps)
 constexpr int kGKP = kGK + 8;   // GEMM LDS row pitch in halfs (144 B)
-constexpr int kCKP = kCK + 8;   // split-pass LDS row pitch in halfs (80 B)
 
 struct GemmDesc {
     TileDesc t;            // segment addressing; line0/count: the line block (count <= 128)
@@ -616,61 +616,75 @@ __device__ __forceinline__ void split_load_chunk(const StageArgs &a, const TileD
     }
 }
 
-__global__ void __launch_bounds__(kGemmThreads) stage_split_kernel(GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) _Float16 th[kGB * kCKP], tl[kGB * kCKP];
+// The split pass runs as two launches over pieces of 128 lines x 64 k (a
+// piece loads its 8192 values at once): stage_umax_kernel takes the largest
+// |u| of every line (an atomic maximum over the pieces of a line block into
+// one of two slot sets, which alternate between stages: the k = 0 pieces clear
+// the other set for the next stage), stage_usplit_kernel scales and splits.
+struct SplitPiece {
+    int lb;  // line block (index of its descriptor)
+    int k0;  // first k of the piece
+};
+
+__global__ void __launch_bounds__(kGemmThreads) stage_umax_kernel(GemmArgs g, const SplitPiece *__restrict__ pieces,
+                                                                  int *umax, long long n_slots, int parity) {
     __shared__ int emax[kGB];
-    __shared__ float escale[kGB];
-    const StageArgs &a = g.st;
-    const GemmDesc gd = g.tiles[blockIdx.x];
+    const SplitPiece pc = pieces[blockIdx.x];
+    const GemmDesc gd = g.tiles[pc.lb];
     const TileDesc &td = gd.t;
-    const int tid = threadIdx.x, K = gd.K, nch = gd.k_pad / kCK;
+    const int tid = threadIdx.x;
+    float v0[16], v1[16];
+    split_load_chunk(g.st, td, gd.K, pc.k0, v0);
+    split_load_chunk(g.st, td, gd.K, pc.k0 + kCK, v1);
     if (tid < kGB) emax[tid] = 0;
     __syncthreads();
-    // largest |u| of every line (two chunks of loads in flight)
-    {
-        float mx[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) mx[i] = 0.f;
-        for (int c = 0; c < nch; c += 2) {
-            float v0[16], v1[16];
-            split_load_chunk(a, td, K, c * kCK, v0);
-            split_load_chunk(a, td, K, (c + 1) * kCK, v1);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) mx[i] = fmaxf(mx[i], fmaxf(fabsf(v0[i]), fabsf(v1[i])));
-        }
-#pragma unroll
-        for (int i = 0; i < 16; ++i) atomicMax(&emax[split_line(td, i)], __float_as_int(mx[i]));
-    }
+    for (int i = 0; i < 16; ++i)
+        atomicMax(&emax[split_line(td, i)], __float_as_int(fmaxf(fabsf(v0[i]), fabsf(v1[i]))));
     __syncthreads();
     if (tid < kGB) {
-        const float m = __int_as_float(emax[tid]);
+        atomicMax(&umax[parity * n_slots + gd.e_off + tid], emax[tid]);
+        if (pc.k0 == 0) umax[(1 - parity) * n_slots + gd.e_off + tid] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(kGemmThreads) stage_usplit_kernel(GemmArgs g, const SplitPiece *__restrict__ pieces,
+                                                                    const int *umax, long long n_slots, int parity) {
+    __shared__ __attribute__((aligned(16))) _Float16 th[kGB * kGKP], tl[kGB * kGKP];
+    __shared__ float escale[kGB];
+    const SplitPiece pc = pieces[blockIdx.x];
+    const GemmDesc gd = g.tiles[pc.lb];
+    const TileDesc &td = gd.t;
+    const int tid = threadIdx.x;
+    float v[2][16];
+    split_load_chunk(g.st, td, gd.K, pc.k0, v[0]);
+    split_load_chunk(g.st, td, gd.K, pc.k0 + kCK, v[1]);
+    if (tid < kGB) {
+        const float m = __int_as_float(umax[parity * n_slots + gd.e_off + tid]);
         const int e = m > 0.f ? min(max(14 - ilogbf(m), -100), 100) : 0;
         escale[tid] = ldexpf(1.f, e);
-        g.uexp[gd.e_off + tid] = e;
+        if (pc.k0 == 0) g.uexp[gd.e_off + tid] = e;
     }
     __syncthreads();
-    _Float16 *Uh = g.umat + gd.u_hi, *Ul = g.umat + gd.u_lo;
-    for (int c = 0; c < nch; ++c) {
-        float v[16];
-        split_load_chunk(a, td, K, c * kCK, v);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            const int line = split_line(td, i), kk = split_k(td, i);
-            const float xs = v[i] * escale[line];
+            const int line = split_line(td, i), kk = h * kCK + split_k(td, i);
+            const float xs = v[h][i] * escale[line];
             const _Float16 hi = (_Float16)xs;
-            th[line * kCKP + kk] = hi;
-            tl[line * kCKP + kk] = (_Float16)(xs - (float)hi);
+            th[line * kGKP + kk] = hi;
+            tl[line * kGKP + kk] = (_Float16)(xs - (float)hi);
         }
-        __syncthreads();
-        // 128 rows x 32 halfs per part: 512 vectors of 8 halfs; two per thread and part
+    __syncthreads();
+    // 128 rows x 64 halfs per part: 1024 vectors of 8 halfs, four per thread and part
+    _Float16 *Uh = g.umat + gd.u_hi, *Ul = g.umat + gd.u_lo;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int vi = tid + j * kGemmThreads, row = vi >> 2, col = (vi & 3) * 8;
-            const size_t o = (size_t)row * gd.k_pad + c * kCK + col;
-            *reinterpret_cast<half8 *>(Uh + o) = *reinterpret_cast<const half8 *>(&th[row * kCKP + col]);
-            *reinterpret_cast<half8 *>(Ul + o) = *reinterpret_cast<const half8 *>(&tl[row * kCKP + col]);
-        }
-        __syncthreads();
+    for (int j = 0; j < 4; ++j) {
+        const int vi = tid + j * kGemmThreads, row = vi >> 3, col = (vi & 7) * 8;
+        const size_t o = (size_t)row * gd.k_pad + pc.k0 + col;
+        *reinterpret_cast<half8 *>(Uh + o) = *reinterpret_cast<const half8 *>(&th[row * kGKP + col]);
+        *reinterpret_cast<half8 *>(Ul + o) = *reinterpret_cast<const half8 *>(&tl[row * kGKP + col]);
     }
 }
 
@@ -808,6 +822,7 @@ struct pstd_ctx {
     std::vector<pstd_plan_domain> plan_domains;
     std::vector<TileDesc> tiles_small, tiles_4096, tiles_8192;
     std::vector<GemmDesc> tiles_gemm, tiles_split;
+    std::vector<SplitPiece> pieces;
     long long u_halfs = 0, u_exps = 0;  // split-input scratch of the GEMM form
     int form = PSTD_FORM_FFT;
     double gemm_flops = 0;   // algorithmic MFMA flops of one stage (3 split products)
@@ -826,6 +841,8 @@ struct pstd_ctx {
     _Float16 *dmat = nullptr;     // derivative matrices of the GEMM form (hi, lo parts)
     _Float16 *umat = nullptr;     // split input lines of the GEMM form (rewritten every stage)
     int *uexp = nullptr;
+    int *umax = nullptr;          // two sets of per-line maxima (float bits), alternating by stage
+    SplitPiece *d_pieces = nullptr;
     ReceiverCell *d_cells = nullptr;
     P0Range *d_ranges = nullptr;
     int n_ranges = 0;
@@ -864,6 +881,10 @@ struct pstd_ctx {
         (void)hipFree(dmat);
         (void)hipFree(umat);
         (void)hipFree(uexp);
+        (void)hipFree(umax);
+        (void)hipFree(d_pieces);
+        umax = nullptr;
+        d_pieces = nullptr;
         d_gemm = d_split = nullptr;
         dmat = umat = nullptr;
         uexp = nullptr;
@@ -959,7 +980,12 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
         ga.umat = ctx->umat;
         ga.uexp = ctx->uexp;
         ga.tiles = ctx->d_split;
-        hipLaunchKernelGGL(stage_split_kernel, dim3(nsp), dim3(kGemmThreads), 0, ctx->stream, ga);
+        const int np = (int)ctx->pieces.size(), parity = k & 1;
+        (void)nsp;
+        hipLaunchKernelGGL(stage_umax_kernel, dim3(np), dim3(kGemmThreads), 0, ctx->stream, ga, ctx->d_pieces,
+                           ctx->umax, ctx->u_exps, parity);
+        hipLaunchKernelGGL(stage_usplit_kernel, dim3(np), dim3(kGemmThreads), 0, ctx->stream, ga, ctx->d_pieces,
+                           ctx->umax, ctx->u_exps, parity);
         ga.tiles = ctx->d_gemm;
         hipLaunchKernelGGL(stage_gemm_kernel, dim3(ng), dim3(kGemmThreads), 0, ctx->stream, ga);
     }
@@ -1159,6 +1185,7 @@ static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_
         gd.k_pad = k_pad;
         gd.d_exp = it->second.second;
         gd.K = K;
+        for (int k0 = 0; k0 < k_pad; k0 += kGK) ctx->pieces.push_back({(int)ctx->tiles_split.size(), k0});
         ctx->tiles_split.push_back(gd);
         for (int n0 = 0; n0 < n_pad; n0 += kGB) {
             gd.n0 = n0;
@@ -1178,6 +1205,7 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     ctx->tiles_8192.clear();
     ctx->tiles_gemm.clear();
     ctx->tiles_split.clear();
+    ctx->pieces.clear();
     ctx->u_halfs = ctx->u_exps = 0;
     ctx->gemm_flops = 0;
     ctx->gemm_cells = 0;
@@ -1362,9 +1390,11 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
             ftab.insert(ftab.end(), w.begin(), w.end());
         }
         g.win_off = wt->second;
-        // GEMM form: segments with enough lines and outputs become 128 x 128 GEMM blocks
+        // GEMM form: segments whose 128 x 128 blocks are at least 3/4 filled (lines
+        // and outputs) become GEMM blocks; thin ones (PML strips) stay FFT tiles
         const int nlines_seg = s.line_end - s.line_begin;
-        if (ctx->form == PSTD_FORM_GEMM_F16 && nlines_seg >= 32 && g.n_out >= 64) {
+        auto filled = [](int n) { return 4 * n >= 3 * ((n + kGB - 1) / kGB * kGB); };
+        if (ctx->form == PSTD_FORM_GEMM_F16 && filled(nlines_seg) && filled(g.n_out)) {
             int rc = add_gemm_tiles(ctx, plan, s, g, dmat, dcache);
             if (rc) return rc;
             continue;
@@ -1472,6 +1502,11 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
                                ctx->stream));
         HIP_TRY(hipMalloc(&ctx->umat, (size_t)ctx->u_halfs * sizeof(_Float16)));
         HIP_TRY(hipMalloc(&ctx->uexp, (size_t)ctx->u_exps * sizeof(int)));
+        HIP_TRY(hipMalloc(&ctx->umax, 2 * (size_t)ctx->u_exps * sizeof(int)));
+        HIP_TRY(hipMemsetAsync(ctx->umax, 0, 2 * (size_t)ctx->u_exps * sizeof(int), ctx->stream));
+        HIP_TRY(hipMalloc(&ctx->d_pieces, ctx->pieces.size() * sizeof(SplitPiece)));
+        HIP_TRY(hipMemcpyAsync(ctx->d_pieces, ctx->pieces.data(), ctx->pieces.size() * sizeof(SplitPiece),
+                               hipMemcpyHostToDevice, ctx->stream));
     }
     HIP_TRY(hipMalloc(&ctx->d_cells, std::max<size_t>(1, cells.size()) * sizeof(ReceiverCell)));
     if (!cells.empty())
