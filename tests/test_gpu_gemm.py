@@ -23,6 +23,22 @@ def rel(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
+def gemm_pair(steps, seed=0):
+    """Two 380 x 382-cell domains side by side (edge absorption 0.3 / 1.0,
+    PML around): both line directions of both domains fill 128 x 128 GEMM
+    blocks, and the shared edge couples them through the side halos."""
+    rng = np.random.RandomState(seed)
+    conf = P.PSTDConfiguration()
+    W.base_settings(conf, steps)
+    conf.Domains = [W.domain(0, 0, 380, 382, {"T": 1.0, "L": 0.3, "B": 1.0, "R": 0.0}),
+                    W.domain(380, 0, 380, 382, {"T": 0.3, "L": 0.0, "B": 1.0, "R": 1.0})]
+    conf.Speakers = [(W._world_pos(300.5), W._world_pos(170.5), 0.0)]
+    for _ in range(4):  # within reach of the wave in a few hundred steps, on both sides of the shared edge
+        conf.Receivers.append((W._world_pos(rng.randint(250, 420) + 0.25), W._world_pos(rng.randint(120, 220) + 0.25),
+                               0.0))
+    return conf
+
+
 def device(conf, form):
     k = P.PSTDKernel(GPU=True)
     k.initialize_kernel(conf)
@@ -40,45 +56,55 @@ def field_errors(k, dev, o, names=("p0", "vx0", "vy0")):
 
 
 @pytest.mark.parametrize("steps", [1, 4])
-def test_lshape_fields_gemm(steps):
-    conf = W.lshape(0, steps)
+def test_pair_fields_gemm(steps):
+    conf = gemm_pair(steps)
     k, dev = device(conf, "gemm")
+    flops, outs = dev.gemm_work()
+    assert flops > 0 and outs > 4 * 380 * 380
     dev.advance(steps)
     o = ob.OracleScene(conf, 64, THREADS)
     o.step(steps)
     errs = field_errors(k, dev, o)
-    print("GEMM form, L-shape", steps, "steps, field rel-L2 vs FP64:", errs)
+    print("GEMM form, two-domain pair", steps, "steps, field rel-L2 vs FP64:", errs)
     assert max(errs.values()) <= 2e-5
 
 
-def test_room_series_gemm():
+def test_pair_series_gemm():
     steps = 300
-    conf = W.room(steps)
+    conf = gemm_pair(steps, seed=1)
     k, dev = device(conf, "gemm")
     got = dev.advance(steps)
     want = ob.OracleScene(conf, 64, THREADS).step(steps)
-    e = rel(got, want)
-    print(f"GEMM form, room {steps} steps: receiver rel-L2 vs FP64 {e:.2e}, max-abs {np.abs(got - want).max():.2e}")
+    _, fft = device(conf, "fft")
+    got_fft = fft.advance(steps)
+    e, e_fft = rel(got, want), rel(got_fft, want)
+    print(f"two-domain pair {steps} steps: receiver rel-L2 vs FP64: GEMM form {e:.2e}, FFT form {e_fft:.2e}; "
+          f"max-abs {np.abs(got - want).max():.2e}")
+    assert np.abs(want).max() > 1e-2
     assert e <= 1e-4
     assert np.abs(got - want).max() <= 5e-6
 
 
-def test_lshape_series_gemm_vs_golden():
+def test_lshape_mixed_forms_vs_golden():
+    """On the L-shape only the velocity X segments fill GEMM blocks (256
+    outputs); the other segments keep the FFT form in the same stage launches."""
     g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lshape_receivers.npz"))
     steps = 600
     conf = W.lshape(0, 2000)
-    k, dev = device(conf, "gemm")
+    _, dev = device(conf, "gemm")
+    assert dev.gemm_work()[0] > 0
     got = dev.advance(steps)
     want = g["fp64"][:steps]
     e = rel(got, want)
-    print(f"GEMM form, L-shape {steps} steps: receiver rel-L2 vs FP64 {e:.2e}, max-abs {np.abs(got - want).max():.2e}")
+    print(f"mixed forms, L-shape {steps} steps: receiver rel-L2 vs FP64 {e:.2e}, max-abs {np.abs(got - want).max():.2e}")
     assert e <= 1e-4
     assert np.abs(got - want).max() <= 5e-6
 
 
 def test_gemm_batch_is_bitwise_single():
-    confs = W.scene_batch(11, 10, n_scenes=3, hi=300)
+    confs = [gemm_pair(10, seed=s) for s in range(2)] + W.scene_batch(11, 10, n_scenes=2, hi=300)
     b = P.BatchSolver(confs, form="gemm")
+    assert b.gemm_work()[0] > 0
     got = b.advance(10)
     for i, conf in enumerate(confs):
         k, dev = device(conf, "gemm")
