@@ -128,27 +128,27 @@ __device__ __forceinline__ void bstore(Rsrc r, int off_bytes, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off_bytes, 0, 0);
 }
 
-// Per-tile geometry shared by the generic (F-independent) parts of a tile.
-struct TileGeom {
-    int F, T, P, logT, logP, layout;
+// Geometry of a derivative tile: F-point lines (run time), NT threads,
+// T = F/8 threads per line pair, P = NT/T pairs; LAYOUT (compile time) 0: X
+// lines (rows; lanes walk positions), 1: Y lines (columns; lanes walk the
+// pairs).  The gather and update code is shared by all F (one copy per
+// layout keeps the kernel small enough for the instruction cache); only the
+// transforms are specialised per F.
+template <int LAYOUT>
+struct TileGeo {
+    int F, T, P, logT, logP;
+    __device__ TileGeo(int F_, int NT) : F(F_), T(F_ >> 3), P(NT / (F_ >> 3)) {
+        logT = __ffs(T) - 1;
+        logP = __ffs(P) - 1;
+    }
+    __device__ int pair(int tid) const { return LAYOUT == 0 ? tid >> logT : tid & (P - 1); }
+    __device__ int lane_t(int tid) const { return LAYOUT == 0 ? tid & (T - 1) : tid >> logP; }
+    // raw staging of the 2P lines in LDS (float): line-major for rows,
+    // position-major for columns, so that the wavefront's lanes stay contiguous
+    __device__ int line_base(int ll) const { return LAYOUT == 0 ? ll << (logT + 3) : ll; }
+    __device__ int pos_step() const { return LAYOUT == 0 ? 1 : 2 * P; }
+    __device__ int stage_at(int ll, int pos) const { return line_base(ll) + pos * pos_step(); }
 };
-
-__device__ __forceinline__ TileGeom tile_geom(int F, int NT, int layout) {
-    TileGeom g;
-    g.F = F;
-    g.T = F >> 3;
-    g.P = NT / g.T;
-    g.logT = __ffs(g.T) - 1;
-    g.logP = __ffs(g.P) - 1;
-    g.layout = layout;
-    return g;
-}
-
-// Raw staging of the lines in LDS (float, line-major for rows, position-major
-// for columns so that the wavefront's lanes stay contiguous).
-__device__ __forceinline__ int stage_at(const TileGeom &g, int ll, int pos) {
-    return g.layout == 0 ? (ll << (g.logT + 3)) + pos : (pos << (g.logP + 1)) + ll;
-}
 
 // The windowed lines of one tile (spatderp3, kernel/core/kernel_functions.cpp:
 // 210-216 pressure, :263-269 velocity):  [dom1 | main | dom3 | 0], where
@@ -164,10 +164,11 @@ struct GatherRegs {
     int hpos1[4], hpos2[4];
 };
 
-template <int NT>
-__device__ __forceinline__ void gather_issue(const StageArgs &a, const TileDesc &td, const TileGeom &g, int p, int t,
-                                             GatherRegs &G) {
+template <int NT, int LAYOUT>
+__device__ __forceinline__ void gather_issue(const StageArgs &a, const TileDesc &td, const TileGeo<LAYOUT> &g, int p,
+                                             int t, GatherRegs &R) {
     const int w = td.wlen, nm = td.n_main, vel = td.vel, count = td.count;
+    __builtin_assume(w > 0 && w <= 32);
     const float *A = vel ? a.vsrc : a.p0a;
     const bool hasB = !vel && a.p0b != nullptr;
     const float *B = hasB ? a.p0b : A;
@@ -175,33 +176,32 @@ __device__ __forceinline__ void gather_issue(const StageArgs &a, const TileDesc 
     const Rsrc rmA = block_rsrc(A, td.m_dom, td.m_size);
     const Rsrc rmB = block_rsrc(B, td.m_dom + (td.m_b - td.m_a), td.m_size - (td.m_b - td.m_a));
     const int la = td.line0 + 2 * p;
-    const int lbase = (td.m_a + la * td.m_ls) * 4, lstep = td.m_ls * 4, pstep = td.m_ps * 4;
+    const int pstep = td.m_ps * 4;
+    const int lbase = (td.m_a + la * td.m_ls) * 4 + (t - w) * pstep, lstep = td.m_ls * 4, mstep = g.T * pstep;
     int off[8][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        const int e = t + (m << g.logT) - w;
+        const int e = t + m * g.T - w;
         const bool ok = e >= 0 && e < nm;
-        off[m][0] = ok && 2 * p < count ? lbase + e * pstep : kOOB;
-        off[m][1] = ok && 2 * p + 1 < count ? lbase + lstep + e * pstep : kOOB;
+        off[m][0] = ok && 2 * p < count ? lbase + m * mstep : kOOB;
+        off[m][1] = ok && 2 * p + 1 < count ? lbase + lstep + m * mstep : kOOB;
     }
     if (hasB) {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) {
+        for (int m = 0; m < 8; ++m)
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                G.ra[m][q] = bload(rmA, off[m][q]);
-                G.rb[m][q] = bload(rmB, off[m][q]);
+                R.ra[m][q] = bload(rmA, off[m][q]);
+                R.rb[m][q] = bload(rmB, off[m][q]);
             }
-        }
     } else {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) {
+        for (int m = 0; m < 8; ++m)
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                G.ra[m][q] = bload(rmA, off[m][q]);
-                G.rb[m][q] = 0.f;
+                R.ra[m][q] = bload(rmA, off[m][q]);
+                R.rb[m][q] = 0.f;
             }
-        }
     }
     // halos: item i of a side is (line ll, offset j), j in slots of 2^wlog >= w;
     // rows: j fastest, columns: ll fastest (coalesced either way)
@@ -213,12 +213,12 @@ __device__ __forceinline__ void gather_issue(const StageArgs &a, const TileDesc 
     const Rsrc r2B = block_rsrc(B, td.s2_dom + (td.s2_b - td.s2_a), td.s2_size - (td.s2_b - td.s2_a));
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        G.hpos1[k] = G.hpos2[k] = -1;
-        G.h1a[k] = G.h1b[k] = G.h2a[k] = G.h2b[k] = 0.f;
+        R.hpos1[k] = R.hpos2[k] = -1;
+        R.h1a[k] = R.h1b[k] = R.h2a[k] = R.h2b[k] = 0.f;
         if (k < nk) {
             const int i = threadIdx.x + k * NT;
-            const int ll = g.layout == 0 ? i >> td.wlog : i & (2 * g.P - 1);
-            const int j = g.layout == 0 ? i & ((1 << td.wlog) - 1) : i >> (g.logP + 1);
+            const int ll = LAYOUT == 0 ? i >> td.wlog : i & (2 * g.P - 1);
+            const int j = LAYOUT == 0 ? i & ((1 << td.wlog) - 1) : i >> (g.logP + 1);
             const bool item = i < n_items && j < w;
             const bool ok = item && ll < count;
             const int line = td.line0 + ll;
@@ -227,46 +227,51 @@ __device__ __forceinline__ void gather_issue(const StageArgs &a, const TileDesc 
                                : kOOB;
             const int o2 = ok && td.s2_len >= 0 ? (td.s2_a + (line + td.s2_line) * td.s2_ls + (j + vel) * td.s2_ps) * 4
                                                 : kOOB;
-            G.h1a[k] = bload(r1A, o1);
-            G.h2a[k] = bload(r2A, o2);
+            R.h1a[k] = bload(r1A, o1);
+            R.h2a[k] = bload(r2A, o2);
             if (hasB) {
-                G.h1b[k] = bload(r1B, o1);
-                G.h2b[k] = bload(r2B, o2);
+                R.h1b[k] = bload(r1B, o1);
+                R.h2b[k] = bload(r2B, o2);
             }
-            G.hpos1[k] = item ? stage_at(g, ll, j) : -1;
-            G.hpos2[k] = item ? stage_at(g, ll, w + nm + j) : -1;
+            R.hpos1[k] = item ? g.stage_at(ll, j) : -1;
+            R.hpos2[k] = item ? g.stage_at(ll, w + nm + j) : -1;
         }
     }
 }
 
 // Stages the raw values, then builds the windowed complex pairs.
-__device__ __forceinline__ void gather_finish(const StageArgs &a, const TileDesc &td, const TileGeom &g, int p, int t,
-                                              const GatherRegs &G, float *stage, float *win, float2 (&v)[8]) {
+template <int LAYOUT>
+__device__ __forceinline__ void gather_finish(const StageArgs &a, const TileDesc &td, const TileGeo<LAYOUT> &g, int p,
+                                              int t, const GatherRegs &R, float *stage, float *win, float2 (&v)[8]) {
     const int w = td.wlen, nm = td.n_main, vel = td.vel;
+    __builtin_assume(w > 0 && w <= 32);
     const float c_s1 = td.c_s1, c_m1 = td.c_m1, c_s2 = td.c_s2, c_m2 = td.c_m2;
+    // LDS addresses: line q of the pair at position pos = lb[q] + pos * ps
+    const int ps = g.pos_step(), lb[2] = {g.line_base(2 * p), g.line_base(2 * p + 1)};
+    const int tps = t * ps, mps = g.T * ps;
     float raw[8][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        const int pos = t + (m << g.logT);
+        const int pos = t + m * g.T;
         const bool mid = pos >= w && pos < w + nm;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            raw[m][q] = G.ra[m][q] + G.rb[m][q];
-            if (mid) stage[stage_at(g, 2 * p + q, pos)] = raw[m][q];
+            raw[m][q] = R.ra[m][q] + R.rb[m][q];
+            if (mid) stage[lb[q] + tps + m * mps] = raw[m][q];
         }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (G.hpos1[k] >= 0) {
-            stage[G.hpos1[k]] = G.h1a[k] + G.h1b[k];
-            stage[G.hpos2[k]] = G.h2a[k] + G.h2b[k];
+        if (R.hpos1[k] >= 0) {
+            stage[R.hpos1[k]] = R.h1a[k] + R.h1b[k];
+            stage[R.hpos2[k]] = R.h2a[k] + R.h2b[k];
         }
     }
     if ((int)threadIdx.x <= 2 * w) win[threadIdx.x] = a.ftab[td.win_off + threadIdx.x];
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        const int i = t + (m << g.logT);
+        const int i = t + m * g.T;
         const bool left = i < w, right = i >= w + nm && i < 2 * w + nm, mid = i >= w && i < w + nm;
         const int j = i - w - nm;
         float out[2] = {0.f, 0.f};
@@ -274,10 +279,11 @@ __device__ __forceinline__ void gather_finish(const StageArgs &a, const TileDesc
             const int mpos = left ? (vel ? w - i : w - 1 - i) : (vel ? nm - 2 - j : nm - 1 - j);
             const float cs = left ? c_s1 : c_s2, cm = left ? c_m1 : c_m2;
             const float wv = win[left ? i : w + 1 + j];
+            const int ri = (w + mpos) * ps, si = tps + m * mps;
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                const float sv = stage[stage_at(g, 2 * p + q, i)];
-                const float refl = stage[stage_at(g, 2 * p + q, w + mpos)];
+                const float sv = stage[lb[q] + si];
+                const float refl = stage[lb[q] + ri];
                 out[q] = __builtin_fmaf(sv, cs, refl * cm) * wv;
             }
         }
@@ -290,9 +296,12 @@ __device__ __forceinline__ void gather_finish(const StageArgs &a, const TileDesc
 // LEAN: the filter is read after the forward transform and twiddles are read
 // per pass from the table `tw` (instead of the preloaded registers `twr`).
 template <int F, int NT, int LAYOUT, bool LEAN>
-__device__ __forceinline__ void derivative_fft(float2 (&v)[8], float2 (&h)[8], const float2 *twr, int p, int t,
-                                               float2 *buf0, float2 *buf1, const float2 *hsrc, const float2 *tw) {
-    constexpr int P = NT / (F / 8);
+__device__ __forceinline__ void derivative_fft(float2 (&v)[8], float2 (&h)[8], const float2 *twr, float2 *buf0,
+                                               float2 *buf1, const float2 *hsrc, const float2 *tw) {
+    constexpr int T = F / 8, P = NT / T;
+    // (p, t) again from the thread index, with F known here
+    const int tid = threadIdx.x;
+    const int p = LAYOUT == 0 ? tid / T : tid % P, t = LAYOUT == 0 ? tid % T : tid / P;
     const float2 *w = LEAN ? tw : twr;
     fft_run<F, P, LAYOUT, false, LEAN>(v, p, t, buf0, buf1, w);
     if constexpr (LEAN) {
@@ -321,20 +330,23 @@ struct UpdateRegs {
     float tnv[8][2], pm[8][2];
 };
 
-__device__ __forceinline__ void prefetch_updates(const StageArgs &a, const TileDesc &td, const TileGeom &g, int p,
-                                                 int t, UpdateRegs &u) {
+template <int LAYOUT>
+__device__ __forceinline__ void prefetch_updates(const StageArgs &a, const TileDesc &td, const TileGeo<LAYOUT> &g,
+                                                 int p, int t, UpdateRegs &u) {
     const int w = td.wlen, n_out = td.n_out;
     const bool ok0 = 2 * p < td.count, ok1 = 2 * p + 1 < td.count;
-    const int base = (td.o_f + (td.line0 + 2 * p) * td.o_ls) * 4, lstep = td.o_ls * 4, pstep = td.o_ps * 4;
+    const int pstep = td.o_ps * 4;
+    const int base = (td.o_f + (td.line0 + 2 * p) * td.o_ls) * 4 + (t - w) * pstep, lstep = td.o_ls * 4,
+              mstep = g.T * pstep;
     const bool use_pml = a.last && td.is_pml;
     const Rsrc rt = block_rsrc(a.tn, td.m_dom, td.m_size);
     const Rsrc rp = block_rsrc(a.pml, td.m_dom, td.m_size);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        const int pos = t + (m << g.logT) - w;
+        const int pos = t + m * g.T - w;
         const bool okp = pos >= 0 && pos < n_out;
-        u.off[m][0] = okp && ok0 ? base + pos * pstep : kOOB;
-        u.off[m][1] = okp && ok1 ? base + lstep + pos * pstep : kOOB;
+        u.off[m][0] = okp && ok0 ? base + m * mstep : kOOB;
+        u.off[m][1] = okp && ok1 ? base + lstep + m * mstep : kOOB;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             u.tnv[m][q] = bload(rt, u.off[m][q]);
@@ -360,19 +372,18 @@ __device__ __forceinline__ void finish_updates(const StageArgs &a, const TileDes
     }
 }
 
-// One derivative tile: NT threads, F = td.F (8 values per thread and line pair).
+// One derivative tile of one layout: NT threads, F = td.F in [FMIN, FMAX].
 // !LEAN issues every load that depends only on the descriptor (filter,
 // twiddles, t_n and PML operands) before the transforms, for the shortest tile
 // latency; LEAN defers them to where they are used, for fewer live registers
 // and more resident waves.  Both compute the same values in the same order.
-template <int NT, int FMIN, int FMAX, bool LEAN>
-__device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDesc &td, float2 *buf0, float2 *buf1) {
-    const TileGeom g = tile_geom(td.F, NT, td.dir);
+template <int NT, int FMIN, int FMAX, bool LEAN, int LAYOUT>
+__device__ __forceinline__ void tile_layout(const StageArgs &a, const TileDesc &td, float2 *buf0, float2 *buf1) {
+    const TileGeo<LAYOUT> g(td.F, NT);
     const int tid = threadIdx.x;
-    const int p = g.layout == 0 ? tid >> g.logT : tid & (g.P - 1);
-    const int t = g.layout == 0 ? tid & (g.T - 1) : tid >> g.logP;
-    GatherRegs G;
-    gather_issue<NT>(a, td, g, p, t, G);
+    const int p = g.pair(tid), t = g.lane_t(tid);
+    GatherRegs R;
+    gather_issue<NT, LAYOUT>(a, td, g, p, t, R);
     float2 h[8];
     float2 twr[LEAN ? 1 : kMaxTw];
     const float2 *tw = a.ctab + td.tw_off;
@@ -392,18 +403,15 @@ __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDe
         }
     }
     UpdateRegs u;
-    if constexpr (!LEAN) prefetch_updates(a, td, g, p, t, u);
+    if constexpr (!LEAN) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
     // buf1 holds the raw lines (2P x F floats) and, after them, the window
     float *stage = reinterpret_cast<float *>(buf1);
     float *win = stage + 16 * NT;
     float2 v[8];
-    gather_finish(a, td, g, p, t, G, stage, win, v);
-#define PSTD_FFT_CASE(FV)                                                                               \
-    case FV:                                                                                            \
-        if constexpr (FMIN <= FV && FMAX >= FV) {                                                       \
-            if (g.layout == 0) derivative_fft<FV, NT, 0, LEAN>(v, h, twr, p, t, buf0, buf1, hsrc, tw);  \
-            else derivative_fft<FV, NT, 1, LEAN>(v, h, twr, p, t, buf0, buf1, hsrc, tw);                \
-        }                                                                                               \
+    gather_finish<LAYOUT>(a, td, g, p, t, R, stage, win, v);
+#define PSTD_FFT_CASE(FV)                                                                                \
+    case FV:                                                                                             \
+        if constexpr (FMIN <= FV && FMAX >= FV) derivative_fft<FV, NT, LAYOUT, LEAN>(v, h, twr, buf0, buf1, hsrc, tw); \
         break;
     switch (td.F) {
         PSTD_FFT_CASE(128)
@@ -416,8 +424,14 @@ __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDe
         default: break;
     }
 #undef PSTD_FFT_CASE
-    if constexpr (LEAN) prefetch_updates(a, td, g, p, t, u);
+    if constexpr (LEAN) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
     finish_updates(a, td, v, u);
+}
+
+template <int NT, int FMIN, int FMAX, bool LEAN>
+__device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDesc &td, float2 *buf0, float2 *buf1) {
+    if (td.dir == 0) tile_layout<NT, FMIN, FMAX, LEAN, 0>(a, td, buf0, buf1);
+    else tile_layout<NT, FMIN, FMAX, LEAN, 1>(a, td, buf0, buf1);
 }
 
 // One extra workgroup of a launch samples the receivers (p0 = px0 + py0 of the
