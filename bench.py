@@ -129,7 +129,7 @@ def main():
                     help="line derivative form: FP32 FFT, or dense matrices on MFMA (FP16 hi+lo split)")
     ap.add_argument("--tile-order", default="auto", choices=["auto", "xcd", "plain"],
                     help="workgroup-to-tile order (default: the autotuner's choice)")
-    ap.add_argument("--flavour", default="auto", choices=["auto", "lean", "prefetch"],
+    ap.add_argument("--flavour", default="auto", choices=["auto", "early", "late", "mid"],
                     help="stage-kernel flavour (default: the autotuner's choice)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -161,7 +161,7 @@ def main():
     if args.tile_order != "auto":
         dev.set_tile_order(args.tile_order == "xcd")
     if args.flavour != "auto":
-        dev.set_kernel_flavour(args.flavour == "lean")
+        dev.set_kernel_flavour(args.flavour)
     tuning = dev.tuning()
     cells = dev.cells_per_stage
 
@@ -206,7 +206,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "scenes": len(confs), "cells_per_stage": cells, "user_cells": user,
                    "domains": len(infos), "rk_stages_per_step": 6, "graph_steps": args.graph_steps,
-                   "launch_shape": {"lean": tuning["lean"], "xcd_order": tuning["xcd_order"],
+                   "launch_shape": {"flavour": ["early", "late", "mid"][tuning["lean"]], "xcd_order": tuning["xcd_order"],
                                     "autotuned": args.tile_order == "auto" and args.flavour == "auto"},
                    "path": ("FP32 FFT (fused LDS line derivative + RK update)" if args.form == "fft" else
                             "dense derivative matrices on MFMA, FP16 hi+lo split operands, FP32 accumulate "

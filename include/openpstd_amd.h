@@ -213,11 +213,12 @@ int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps);
  *  - kernel flavour: 0 loads every descriptor-dependent operand (filter,
  *    twiddles, t_n) before the transforms (short tiles, latency-bound scenes),
  *    1 loads them where they are used (fewer registers, more resident waves,
- *    throughput-bound scenes).
+ *    throughput-bound scenes), 2 loads filter and twiddles where used and t_n
+ *    as soon as the lines have arrived.
  * pstd_ctx_get_tuning reports the choice and its timed GPU ms per stage. */
 int pstd_ctx_set_tile_order(pstd_ctx *ctx, int xcd_contiguous);
-int pstd_ctx_set_kernel_flavour(pstd_ctx *ctx, int lean);
-int pstd_ctx_get_tuning(const pstd_ctx *ctx, int32_t *lean, int32_t *xcd_contiguous, double *stage_ms);
+int pstd_ctx_set_kernel_flavour(pstd_ctx *ctx, int flavour);
+int pstd_ctx_get_tuning(const pstd_ctx *ctx, int32_t *flavour, int32_t *xcd_contiguous, double *stage_ms);
 /* Work of the GEMM-form segments in one RK stage: algorithmic MFMA flops
  * (3 split products x 2 x K x outputs x lines) and line outputs. */
 int pstd_ctx_gemm_work(const pstd_ctx *ctx, double *flops_per_stage, long long *outputs_per_stage);

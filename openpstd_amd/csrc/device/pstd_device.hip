@@ -373,11 +373,15 @@ __device__ __forceinline__ void finish_updates(const StageArgs &a, const TileDes
 }
 
 // One derivative tile of one layout: NT threads, F = td.F in [FMIN, FMAX].
-// !LEAN issues every load that depends only on the descriptor (filter,
-// twiddles, t_n and PML operands) before the transforms, for the shortest tile
-// latency; LEAN defers them to where they are used, for fewer live registers
-// and more resident waves.  Both compute the same values in the same order.
-template <int NT, int FMIN, int FMAX, bool LEAN, int LAYOUT>
+// Flavours differ only in when the loads that depend on nothing but the
+// descriptor (filter, twiddles, t_n and PML operands) are issued:
+//  kEarly: before the gather completes (shortest tile latency, most registers);
+//  kLate:  where they are used (fewest live registers, most resident waves);
+//  kMid:   filter and twiddles where used, t_n as soon as the lines arrived,
+//          so its latency hides behind the transforms.
+// All compute the same values in the same order.
+constexpr int kEarly = 0, kLate = 1, kMid = 2;
+template <int NT, int FMIN, int FMAX, int FL, int LAYOUT>
 __device__ __forceinline__ void tile_layout(const StageArgs &a, const TileDesc &td, float2 *buf0, float2 *buf1) {
     const TileGeo<LAYOUT> g(td.F, NT);
     const int tid = threadIdx.x;
@@ -385,6 +389,7 @@ __device__ __forceinline__ void tile_layout(const StageArgs &a, const TileDesc &
     GatherRegs R;
     gather_issue<NT, LAYOUT>(a, td, g, p, t, R);
     float2 h[8];
+    constexpr bool LEAN = FL != kEarly;
     float2 twr[LEAN ? 1 : kMaxTw];
     const float2 *tw = a.ctab + td.tw_off;
     const float2 *hsrc = a.ctab + td.h_off;
@@ -403,12 +408,13 @@ __device__ __forceinline__ void tile_layout(const StageArgs &a, const TileDesc &
         }
     }
     UpdateRegs u;
-    if constexpr (!LEAN) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
+    if constexpr (FL == kEarly) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
     // buf1 holds the raw lines (2P x F floats) and, after them, the window
     float *stage = reinterpret_cast<float *>(buf1);
     float *win = stage + 16 * NT;
     float2 v[8];
     gather_finish<LAYOUT>(a, td, g, p, t, R, stage, win, v);
+    if constexpr (FL == kMid) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
 #define PSTD_FFT_CASE(FV)                                                                                \
     case FV:                                                                                             \
         if constexpr (FMIN <= FV && FMAX >= FV) derivative_fft<FV, NT, LAYOUT, LEAN>(v, h, twr, buf0, buf1, hsrc, tw); \
@@ -424,14 +430,14 @@ __device__ __forceinline__ void tile_layout(const StageArgs &a, const TileDesc &
         default: break;
     }
 #undef PSTD_FFT_CASE
-    if constexpr (LEAN) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
+    if constexpr (FL == kLate) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
     finish_updates(a, td, v, u);
 }
 
-template <int NT, int FMIN, int FMAX, bool LEAN>
+template <int NT, int FMIN, int FMAX, int FL>
 __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDesc &td, float2 *buf0, float2 *buf1) {
-    if (td.dir == 0) tile_layout<NT, FMIN, FMAX, LEAN, 0>(a, td, buf0, buf1);
-    else tile_layout<NT, FMIN, FMAX, LEAN, 1>(a, td, buf0, buf1);
+    if (td.dir == 0) tile_layout<NT, FMIN, FMAX, FL, 0>(a, td, buf0, buf1);
+    else tile_layout<NT, FMIN, FMAX, FL, 1>(a, td, buf0, buf1);
 }
 
 // One extra workgroup of a launch samples the receivers (p0 = px0 + py0 of the
@@ -484,7 +490,7 @@ __device__ __forceinline__ int tile_index(int b, int n, int remap) {
 // two LDS buffers of P * F = 8 * NT complex values each (padded)
 constexpr int kSmallBuf = lds_padded(8 * kSmallThreads);
 
-template <bool LEAN>
+template <int FL>
 __global__ void __launch_bounds__(kSmallThreads) stage_small_kernel(StageArgs a) {
     __shared__ float2 lds[2 * kSmallBuf];
     // a copy taken before any store: the descriptor stays in scalar registers
@@ -497,14 +503,14 @@ __global__ void __launch_bounds__(kSmallThreads) stage_small_kernel(StageArgs a)
         passthrough_tile(a, td);
         return;
     }
-    derivative_tile<kSmallThreads, 128, 2048, LEAN>(a, td, lds, lds + kSmallBuf);
+    derivative_tile<kSmallThreads, 128, 2048, FL>(a, td, lds, lds + kSmallBuf);
 }
 
-template <int F, bool LEAN>
+template <int F, int FL>
 __global__ void __launch_bounds__(F / 8) stage_big_kernel(StageArgs a) {
     extern __shared__ float2 lds_dyn[];
     const TileDesc td = a.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
-    derivative_tile<F / 8, F, F, LEAN>(a, td, lds_dyn, lds_dyn + LdsMap<F, 1, 0>::SIZE);
+    derivative_tile<F / 8, F, F, FL>(a, td, lds_dyn, lds_dyn + LdsMap<F, 1, 0>::SIZE);
 }
 
 __global__ void receiver_kernel(const float *__restrict__ arena, const ReceiverCell *__restrict__ cells, int nrec,
@@ -702,23 +708,27 @@ __global__ void __launch_bounds__(kGemmThreads) stage_usplit_kernel(GemmArgs g, 
     }
 }
 
+// MK: k per LDS chunk (32: 40 KB of LDS, three blocks per CU; 64: 74 KB, two).
+template <int MK>
 __global__ void __launch_bounds__(kGemmThreads) stage_gemm_kernel(GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) _Float16 uh[kGB * kGKP], ul[kGB * kGKP], dh[kGB * kGKP], dl[kGB * kGKP];
+    constexpr int MKP = MK + 8;        // LDS row pitch in halfs
+    constexpr int NV = MK / 8 / 2;     // 16-byte vectors per thread and part (128 rows x MK halfs)
+    __shared__ __attribute__((aligned(16))) _Float16 uh[kGB * MKP], ul[kGB * MKP], dh[kGB * MKP], dl[kGB * MKP];
     __shared__ int uexp[kGB];
     const StageArgs &a = g.st;
     const GemmDesc gd = g.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
     const TileDesc &td = gd.t;
-    const int tid = threadIdx.x, nch = gd.k_pad / kGK;
+    const int tid = threadIdx.x, nch = gd.k_pad / MK;
     if (tid < kGB) uexp[tid] = g.uexp[gd.e_off + tid];
-    // operand chunks: 128 rows x 64 halfs per part = 1024 vectors; four per thread and part
+    // operand chunks: 128 rows x MK halfs per part, NV vectors of 8 halfs per thread and part
     const _Float16 *Uh = g.umat + gd.u_hi, *Ul = g.umat + gd.u_lo;
     const _Float16 *Dh = g.dmat + gd.d_hi + (size_t)gd.n0 * gd.k_pad, *Dl = g.dmat + gd.d_lo + (size_t)gd.n0 * gd.k_pad;
-    half8 pu_h[4], pu_l[4], pd_h[4], pd_l[4];
+    half8 pu_h[NV], pu_l[NV], pd_h[NV], pd_l[NV];
     auto load_chunk = [&](int c) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int vi = tid + j * kGemmThreads, row = vi >> 3, col = (vi & 7) * 8;
-            const size_t o = (size_t)row * gd.k_pad + c * kGK + col;
+        for (int j = 0; j < NV; ++j) {
+            const int vi = tid + j * kGemmThreads, row = vi / (MK / 8), col = (vi % (MK / 8)) * 8;
+            const size_t o = (size_t)row * gd.k_pad + c * MK + col;
             pu_h[j] = *reinterpret_cast<const half8 *>(Uh + o);
             pu_l[j] = *reinterpret_cast<const half8 *>(Ul + o);
             pd_h[j] = *reinterpret_cast<const half8 *>(Dh + o);
@@ -740,27 +750,27 @@ __global__ void __launch_bounds__(kGemmThreads) stage_gemm_kernel(GemmArgs g) {
     for (int c = 0; c < nch; ++c) {
         __syncthreads();  // the previous chunk's MFMA reads are done
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int vi = tid + j * kGemmThreads, row = vi >> 3, col = (vi & 7) * 8;
-            *reinterpret_cast<half8 *>(&uh[row * kGKP + col]) = pu_h[j];
-            *reinterpret_cast<half8 *>(&ul[row * kGKP + col]) = pu_l[j];
-            *reinterpret_cast<half8 *>(&dh[row * kGKP + col]) = pd_h[j];
-            *reinterpret_cast<half8 *>(&dl[row * kGKP + col]) = pd_l[j];
+        for (int j = 0; j < NV; ++j) {
+            const int vi = tid + j * kGemmThreads, row = vi / (MK / 8), col = (vi % (MK / 8)) * 8;
+            *reinterpret_cast<half8 *>(&uh[row * MKP + col]) = pu_h[j];
+            *reinterpret_cast<half8 *>(&ul[row * MKP + col]) = pu_l[j];
+            *reinterpret_cast<half8 *>(&dh[row * MKP + col]) = pd_h[j];
+            *reinterpret_cast<half8 *>(&dl[row * MKP + col]) = pd_l[j];
         }
         __syncthreads();
         if (c + 1 < nch) load_chunk(c + 1);
 #pragma unroll
-        for (int s = 0; s < kGK / 16; ++s) {
+        for (int s = 0; s < MK / 16; ++s) {
             const int kk = s * 16 + (lane >> 5) * 8;
             half8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 const int row = wr * 64 + i * 32 + (lane & 31);
-                ah[i] = *reinterpret_cast<const half8 *>(&Rh[row * kGKP + kk]);
-                al[i] = *reinterpret_cast<const half8 *>(&Rl[row * kGKP + kk]);
+                ah[i] = *reinterpret_cast<const half8 *>(&Rh[row * MKP + kk]);
+                al[i] = *reinterpret_cast<const half8 *>(&Rl[row * MKP + kk]);
                 const int col = wc * 64 + i * 32 + (lane & 31);
-                bh[i] = *reinterpret_cast<const half8 *>(&Qh[col * kGKP + kk]);
-                bl[i] = *reinterpret_cast<const half8 *>(&Ql[col * kGKP + kk]);
+                bh[i] = *reinterpret_cast<const half8 *>(&Qh[col * MKP + kk]);
+                bl[i] = *reinterpret_cast<const half8 *>(&Ql[col * MKP + kk]);
             }
 #pragma unroll
             for (int i = 0; i < 2; ++i)
@@ -839,6 +849,7 @@ struct pstd_ctx {
     std::vector<SplitPiece> pieces;
     long long u_halfs = 0, u_exps = 0;  // split-input scratch of the GEMM form
     int form = PSTD_FORM_FFT;
+    int gemm_mk = 64;        // k per LDS chunk of the GEMM kernel (autotuned: 32 or 64)
     double gemm_flops = 0;   // algorithmic MFMA flops of one stage (3 split products)
     long long gemm_cells = 0;
     size_t arena_floats = 0;
@@ -865,7 +876,7 @@ struct pstd_ctx {
     int steps_done = 0;
     bool p0_explicit = true;
     int xcd_remap = 1;
-    int lean = 0;  // stage kernel flavour (derivative_tile LEAN)
+    int lean = 0;  // stage kernel flavour (kEarly, kLate, kMid)
     float tuned_stage_ms = 0;
     // hipGraph of `graph_steps` whole time steps (0: launches are issued one by one)
     int graph_steps = 0;
@@ -968,23 +979,26 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
     a.last = k == 5;
     a.remap = ctx->xcd_remap;
     const int ns = (int)ctx->tiles_small.size(), n4 = (int)ctx->tiles_4096.size(), n8 = (int)ctx->tiles_8192.size();
-    const bool lean = ctx->lean != 0;
+    const int fl = ctx->lean;
     if (ns) {
         a.tiles = ctx->d_tiles;
-        if (lean) hipLaunchKernelGGL(stage_small_kernel<true>, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
-        else hipLaunchKernelGGL(stage_small_kernel<false>, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
+        if (fl == kLate) hipLaunchKernelGGL(stage_small_kernel<kLate>, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
+        else if (fl == kMid) hipLaunchKernelGGL(stage_small_kernel<kMid>, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
+        else hipLaunchKernelGGL(stage_small_kernel<kEarly>, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
     }
     if (n4) {
         a.tiles = ctx->d_tiles + ns;
         size_t lds = 2 * sizeof(float2) * LdsMap<4096, 1, 0>::SIZE;
-        if (lean) hipLaunchKernelGGL((stage_big_kernel<4096, true>), dim3(n4), dim3(512), lds, ctx->stream, a);
-        else hipLaunchKernelGGL((stage_big_kernel<4096, false>), dim3(n4), dim3(512), lds, ctx->stream, a);
+        if (fl == kLate) hipLaunchKernelGGL((stage_big_kernel<4096, kLate>), dim3(n4), dim3(512), lds, ctx->stream, a);
+        else if (fl == kMid) hipLaunchKernelGGL((stage_big_kernel<4096, kMid>), dim3(n4), dim3(512), lds, ctx->stream, a);
+        else hipLaunchKernelGGL((stage_big_kernel<4096, kEarly>), dim3(n4), dim3(512), lds, ctx->stream, a);
     }
     if (n8) {
         a.tiles = ctx->d_tiles + ns + n4;
         size_t lds = 2 * sizeof(float2) * LdsMap<8192, 1, 0>::SIZE;
-        if (lean) hipLaunchKernelGGL((stage_big_kernel<8192, true>), dim3(n8), dim3(1024), lds, ctx->stream, a);
-        else hipLaunchKernelGGL((stage_big_kernel<8192, false>), dim3(n8), dim3(1024), lds, ctx->stream, a);
+        if (fl == kLate) hipLaunchKernelGGL((stage_big_kernel<8192, kLate>), dim3(n8), dim3(1024), lds, ctx->stream, a);
+        else if (fl == kMid) hipLaunchKernelGGL((stage_big_kernel<8192, kMid>), dim3(n8), dim3(1024), lds, ctx->stream, a);
+        else hipLaunchKernelGGL((stage_big_kernel<8192, kEarly>), dim3(n8), dim3(1024), lds, ctx->stream, a);
     }
     const int ng = (int)ctx->tiles_gemm.size(), nsp = (int)ctx->tiles_split.size();
     if (ng) {
@@ -1001,7 +1015,8 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
         hipLaunchKernelGGL(stage_usplit_kernel, dim3(np), dim3(kGemmThreads), 0, ctx->stream, ga, ctx->d_pieces,
                            ctx->umax, ctx->u_exps, parity);
         ga.tiles = ctx->d_gemm;
-        hipLaunchKernelGGL(stage_gemm_kernel, dim3(ng), dim3(kGemmThreads), 0, ctx->stream, ga);
+        if (ctx->gemm_mk == 32) hipLaunchKernelGGL(stage_gemm_kernel<32>, dim3(ng), dim3(kGemmThreads), 0, ctx->stream, ga);
+        else hipLaunchKernelGGL(stage_gemm_kernel<64>, dim3(ng), dim3(kGemmThreads), 0, ctx->stream, ga);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -1063,10 +1078,15 @@ int pstd_ctx_create(int device, pstd_ctx **out) {
     // the long-line classes need more than 64 KiB of LDS per workgroup
     const size_t lds8 = 2 * sizeof(float2) * LdsMap<8192, 1, 0>::SIZE, lds4 = 2 * sizeof(float2) * LdsMap<4096, 1, 0>::SIZE;
     const hipFuncAttribute attr = hipFuncAttributeMaxDynamicSharedMemorySize;
-    if (hipFuncSetAttribute((const void *)stage_big_kernel<8192, false>, attr, lds8) != hipSuccess ||
-        hipFuncSetAttribute((const void *)stage_big_kernel<8192, true>, attr, lds8) != hipSuccess ||
-        hipFuncSetAttribute((const void *)stage_big_kernel<4096, false>, attr, lds4) != hipSuccess ||
-        hipFuncSetAttribute((const void *)stage_big_kernel<4096, true>, attr, lds4) != hipSuccess) {
+    const void *big8[3] = {(const void *)stage_big_kernel<8192, kEarly>, (const void *)stage_big_kernel<8192, kLate>,
+                           (const void *)stage_big_kernel<8192, kMid>};
+    const void *big4[3] = {(const void *)stage_big_kernel<4096, kEarly>, (const void *)stage_big_kernel<4096, kLate>,
+                           (const void *)stage_big_kernel<4096, kMid>};
+    bool lds_ok = true;
+    for (int f = 0; f < 3; ++f)
+        lds_ok = lds_ok && hipFuncSetAttribute(big8[f], attr, lds8) == hipSuccess &&
+                 hipFuncSetAttribute(big4[f], attr, lds4) == hipSuccess;
+    if (!lds_ok) {
         ctx->err = "cannot reserve LDS for the long-line kernels";
     }
     *out = ctx;
@@ -1108,26 +1128,42 @@ static int enqueue_steps(pstd_ctx *ctx, int chunk) {
 // Zero fields stay exactly zero under the update, and the arenas are cleared
 // again afterwards, so tuning leaves no trace in the results.
 static int autotune(pstd_ctx *ctx) {
-    static const int cand[4][2] = {{0, 1}, {1, 1}, {0, 0}, {1, 0}};  // (lean, xcd order)
+    static const int cand[6][2] = {{kLate, 1}, {kMid, 1}, {kEarly, 1}, {kLate, 0}, {kMid, 0}, {kEarly, 0}};  // (flavour, xcd order)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    float best = 0;
-    int bi = 0, rc = PSTD_OK;
-    for (int c = 0; c < 4 && rc == PSTD_OK; ++c) {
-        ctx->lean = cand[c][0];
-        ctx->xcd_remap = cand[c][1];
-        rc = enqueue_steps(ctx, 1);
-        float ms = 0;
+    // one warm-up step, then GPU ms of three steps
+    auto time_steps = [&](float &ms) -> int {
+        int rc = enqueue_steps(ctx, 1);
         if (rc == PSTD_OK && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = PSTD_ERR_DEVICE;
         if (rc == PSTD_OK) rc = enqueue_steps(ctx, 3);
         if (rc == PSTD_OK && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = PSTD_ERR_DEVICE;
         if (rc == PSTD_OK && hipEventSynchronize(e1) != hipSuccess) rc = PSTD_ERR_DEVICE;
         if (rc == PSTD_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PSTD_ERR_DEVICE;
+        return rc;
+    };
+    float best = 0;
+    int bi = 0, rc = PSTD_OK;
+    for (int c = 0; c < 6 && rc == PSTD_OK; ++c) {
+        ctx->lean = cand[c][0];
+        ctx->xcd_remap = cand[c][1];
+        float ms = 0;
+        rc = time_steps(ms);
         if (rc == PSTD_OK && (c == 0 || ms < best)) {
             best = ms;
             bi = c;
         }
+    }
+    if (rc == PSTD_OK) {
+        ctx->lean = cand[bi][0];
+        ctx->xcd_remap = cand[bi][1];
+    }
+    if (rc == PSTD_OK && !ctx->tiles_gemm.empty()) {  // GEMM chunk depth, with the chosen flavour and order
+        ctx->gemm_mk = 32;
+        float ms = 0;
+        rc = time_steps(ms);
+        if (rc == PSTD_OK && ms < best) best = ms;
+        else ctx->gemm_mk = 64;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1135,8 +1171,6 @@ static int autotune(pstd_ctx *ctx) {
         if (ctx->err.empty()) ctx->err = "autotune: HIP event failure";
         return rc;
     }
-    ctx->lean = cand[bi][0];
-    ctx->xcd_remap = cand[bi][1];
     ctx->tuned_stage_ms = best / 18.0;
     HIP_TRY(hipMemsetAsync(ctx->arena, 0, 4 * (size_t)ctx->arena_floats * sizeof(float), ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1748,12 +1782,12 @@ int pstd_ctx_set_form(pstd_ctx *ctx, int form) {
     return PSTD_OK;
 }
 
-int pstd_ctx_set_kernel_flavour(pstd_ctx *ctx, int lean) {
-    if (!ctx) return PSTD_ERR_INVALID;
+int pstd_ctx_set_kernel_flavour(pstd_ctx *ctx, int flavour) {
+    if (!ctx || flavour < kEarly || flavour > kMid) return PSTD_ERR_INVALID;
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->drop_graph();
-    ctx->lean = lean ? 1 : 0;
+    ctx->lean = flavour;
     return PSTD_OK;
 }
 

@@ -232,11 +232,15 @@ class _Context:
     def set_tile_order(self, xcd_contiguous):
         _check(lib().pstd_ctx_set_tile_order(self.ctx, int(bool(xcd_contiguous))), self._err)
 
-    def set_kernel_flavour(self, lean):
-        _check(lib().pstd_ctx_set_kernel_flavour(self.ctx, int(bool(lean))), self._err)
+    FLAVOURS = {"early": 0, "late": 1, "mid": 2}
+
+    def set_kernel_flavour(self, flavour):
+        """Stage-kernel flavour: "early", "late" or "mid" (or 0, 1, 2)."""
+        f = self.FLAVOURS.get(flavour, flavour)
+        _check(lib().pstd_ctx_set_kernel_flavour(self.ctx, int(f)), self._err)
 
     def tuning(self):
-        """The autotuner's choice: {"lean", "xcd_order", "stage_ms"}."""
+        """The autotuner's choice: {"lean" (flavour 0 early, 1 late, 2 mid), "xcd_order", "stage_ms"}."""
         lean, xcd = C.c_int32(), C.c_int32()
         ms = C.c_double()
         _check(lib().pstd_ctx_get_tuning(self.ctx, C.byref(lean), C.byref(xcd), C.byref(ms)), self._err)

@@ -101,15 +101,20 @@ def lshape(seed=0, steps=2000, absorptions=(0.0, 0.5, 1.0)):
     return conf
 
 
-def square(seed=0, steps=1000, n=4096, n_receivers=16):
-    """One n x n domain, PML on all sides, centre source, seeded receivers."""
+def square(seed=0, steps=1000, n=4096, n_receivers=16, receiver_radius=None):
+    """One n x n domain, PML on all sides, centre source, seeded receivers
+    (anywhere in the domain, or within `receiver_radius` cells of the source)."""
     rng = np.random.RandomState(seed)
     conf = PSTDConfiguration()
     base_settings(conf, steps)
     conf.Domains.append(domain(0, 0, n, n, 1.0))
     conf.Speakers.append((_world_pos(n / 2), _world_pos(n / 2), 0.0))
     for _ in range(n_receivers):
-        rx, ry = rng.randint(1, n - 1), rng.randint(1, n - 1)
+        if receiver_radius is None:
+            rx, ry = rng.randint(1, n - 1), rng.randint(1, n - 1)
+        else:
+            r = receiver_radius
+            rx, ry = n // 2 + rng.randint(-r, r + 1), n // 2 + rng.randint(-r, r + 1)
         conf.Receivers.append((_world_pos(rx + 0.25), _world_pos(ry + 0.25), 0.0))
     return conf
 

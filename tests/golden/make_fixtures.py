@@ -5,9 +5,11 @@
      window_w32_pe70.json      test/Kernel/kernel_functions.cpp:147
      kat_reference.json        next_2_power, get_rho_array, wavenumbers, Gaussian
 2. Long-run oracle series (FP64 CPU oracle), too slow to recompute in a test:
-     room_receivers.npz, lshape_receivers.npz
+     room_receivers.npz, lshape_receivers.npz, square1024_receivers.npz
+     (a 1024 x 1024-cell domain whose lines fill the GEMM form's blocks, 300
+     steps, 16 receivers within 120 cells of the source)
 
-    python tests/golden/make_fixtures.py [--kat] [--series]
+    python tests/golden/make_fixtures.py [--kat] [--series] [--square]
 """
 import json
 import os
@@ -52,10 +54,13 @@ def kat():
     print("wrote KAT fixtures")
 
 
-def series():
+def series(which=("room", "lshape")):
     from oracle import binding as ob
     from openpstd_amd import workloads as W
-    for name, conf, steps in (("room", W.room(500), 500), ("lshape", W.lshape(0, 2000), 2000)):
+    cases = {"room": (W.room(500), 500), "lshape": (W.lshape(0, 2000), 2000),
+             "square1024": (W.square(0, 300, n=1024, n_receivers=16, receiver_radius=120), 300)}
+    for name in which:
+        conf, steps = cases[name]
         out = {}
         for prec in (64, 32):
             o = ob.OracleScene(conf, prec, threads=os.cpu_count() or 1)
@@ -70,3 +75,5 @@ if __name__ == "__main__":
         kat()
     if "--series" in args:
         series()
+    if "--square" in args:
+        series(("square1024",))

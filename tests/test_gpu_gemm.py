@@ -111,3 +111,23 @@ def test_gemm_batch_is_bitwise_single():
         assert np.array_equal(b.scene_samples(got, i), dev.advance(10)), i
         for d in range(k.domain_count()):
             assert np.array_equal(b.field(i, d, "p0"), dev.field(d, "p0")), (i, d)
+
+
+def test_square1024_forms_vs_golden():
+    """A 1024 x 1024-cell domain (GEMM blocks filled in both directions) against
+    the committed FP64 oracle series (300 steps, 16 receivers near the source),
+    in both forms; the GEMM form within the 1e-4 bar."""
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "square1024_receivers.npz"))
+    conf = W.square(0, 300, n=1024, n_receivers=16, receiver_radius=120)
+    errs = {}
+    for form in ("gemm", "fft"):
+        _, dev = device(conf, form)
+        if form == "gemm":
+            assert dev.gemm_work()[0] > 0
+        got = dev.advance(300)
+        errs[form] = (rel(got, g["fp64"]), np.abs(got - g["fp64"]).max())
+    e32 = rel(g["fp32"], g["fp64"])
+    print(f"1024^2, 300 steps, receiver rel-L2 / max-abs vs FP64: GEMM {errs['gemm'][0]:.2e} / {errs['gemm'][1]:.2e}, "
+          f"FFT {errs['fft'][0]:.2e} / {errs['fft'][1]:.2e}, CPU FP32 oracle {e32:.2e}")
+    assert errs["gemm"][0] <= 1e-4 and errs["gemm"][1] <= 5e-6
+    assert errs["fft"][0] <= 1e-4

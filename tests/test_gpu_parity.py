@@ -168,6 +168,21 @@ def test_long_lines(n):
     compare_fields(k, dev, o, 5e-6)
 
 
+@pytest.mark.parametrize("n", [2000, 4100])
+def test_long_columns(n):
+    """F = 4096 and F = 8192 columns (the long-line kernels, Y layout)."""
+    conf = P.PSTDConfiguration()
+    W.base_settings(conf, 4)
+    conf.Domains = [W.domain(0, 0, 60, n, 1.0)]
+    conf.Speakers = [(W._world_pos(30.0), W._world_pos(n / 2), 0.0)]
+    conf.Receivers = [(W._world_pos(33.25), W._world_pos(n / 2 + 7.25), 0.0)]
+    k, dev = device(conf)
+    dev.advance(4)
+    o = ob.OracleScene(conf, 64, THREADS)
+    o.step(4)
+    compare_fields(k, dev, o, 5e-6)
+
+
 def test_run_callbacks_contract():
     """PSTDKernel.run: the callback sequence of SingleThreadSolver (kernel/Solver.cpp:62-93)."""
     steps = 12
