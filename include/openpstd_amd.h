@@ -213,8 +213,8 @@ int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps);
  *  - kernel flavour: 0 loads every descriptor-dependent operand (filter,
  *    twiddles, t_n) before the transforms (short tiles, latency-bound scenes),
  *    1 loads them where they are used (fewer registers, more resident waves,
- *    throughput-bound scenes), 2 loads filter and twiddles where used and t_n
- *    as soon as the lines have arrived.
+ *    throughput-bound scenes), 2 loads filter and twiddles up front and t_n
+ *    where used.
  * pstd_ctx_get_tuning reports the choice and its timed GPU ms per stage. */
 int pstd_ctx_set_tile_order(pstd_ctx *ctx, int xcd_contiguous);
 int pstd_ctx_set_kernel_flavour(pstd_ctx *ctx, int flavour);

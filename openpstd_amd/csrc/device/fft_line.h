@@ -135,10 +135,10 @@ __device__ __forceinline__ void fft_load_twiddles(float2 *w, int t, const float2
 // One Stockham pass.  v: natural layout input.  Non-last passes write their
 // outputs to `out`; the last pass leaves them in v (natural layout).
 // TWG = false: w holds the thread's preloaded twiddle registers
-// (fft_load_twiddles); TWG = true: w is the twiddle table exp(-2 pi i k/F) in
-// global memory and each pass reads the factors it needs when it needs them
-// (fewer live registers, more waves per SIMD).
-template <int F, int P, int LAYOUT, int PASS, bool INV, bool TWG>
+// (fft_load_twiddles, or with PERPASS only this pass's, fft_pass_twiddles);
+// TWG = true: w is the twiddle table exp(-2 pi i k/F) in global memory and
+// the pass reads the factors it needs when it needs them.
+template <int F, int P, int LAYOUT, int PASS, bool INV, bool TWG, bool PERPASS = false>
 __device__ __forceinline__ void fft_pass(float2 (&v)[8], int p, int t, float2 *out, const float2 *w) {
     constexpr int T = F / 8;
     constexpr int R = fft_radix(F, PASS);
@@ -157,7 +157,8 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[8], int p, int t, float2 *o
         if constexpr (NS > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                float2 wv = TWG ? w[(r * k) * (F / (NS * R))] : w[base_tw + s * (R - 1) + (r - 1)];
+                float2 wv = TWG ? w[(r * k) * (F / (NS * R))]
+                                : w[(PERPASS ? 0 : base_tw) + s * (R - 1) + (r - 1)];
                 if (INV) wv.y = -wv.y;
                 a[r] = cmul(a[r], wv);
             }
@@ -180,6 +181,43 @@ __device__ __forceinline__ void lds_load(float2 (&v)[8], int p, int t, const flo
     using L = LdsMap<F, P, LAYOUT>;
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = in[L::at(p, t + m * T)];
+}
+
+// Twiddles of one pass for thread t, read from the table exp(-2 pi i k/F):
+// NB butterflies x (R-1) factors, at most 7.
+constexpr int kTwPass = 7;
+template <int F, int PASS>
+__device__ __forceinline__ void fft_pass_twiddles(float2 (&w)[kTwPass], int t, const float2 *__restrict__ tw) {
+    if constexpr (PASS >= 1 && PASS < fft_npasses(F)) {
+        constexpr int T = F / 8;
+        constexpr int R = fft_radix(F, PASS);
+        constexpr int NS = fft_span(F, PASS);
+        constexpr int NB = 8 / R;
+#pragma unroll
+        for (int s = 0; s < NB; ++s) {
+            const int k = (t + s * T) & (NS - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[s * (R - 1) + (r - 1)] = tw[(r * k) * (F / (NS * R))];
+        }
+    }
+}
+
+// Transform with the twiddles of each pass loaded one pass ahead, into
+// registers: the loads of pass p+1 are in flight while pass p computes and
+// exchanges.  wcur: this pass's twiddles (unused by pass 0).
+template <int F, int P, int LAYOUT, bool INV, int PASS = 0>
+__device__ __forceinline__ void fft_run_ahead(float2 (&v)[8], int p, int t, float2 *buf0, float2 *buf1,
+                                              const float2 *tw, const float2 (&wcur)[kTwPass]) {
+    constexpr int NP = fft_npasses(F);
+    float2 wnext[kTwPass];
+    if constexpr (PASS + 1 < NP) fft_pass_twiddles<F, PASS + 1>(wnext, t, tw);
+    float2 *out = (PASS & 1) ? buf1 : buf0;
+    fft_pass<F, P, LAYOUT, PASS, INV, false, true>(v, p, t, out, wcur);
+    if constexpr (PASS + 1 < NP) {
+        __syncthreads();
+        lds_load<F, P, LAYOUT>(v, p, t, out);
+        fft_run_ahead<F, P, LAYOUT, INV, PASS + 1>(v, p, t, buf0, buf1, tw, wnext);
+    }
 }
 
 // Full transform (forward if !INV) on natural-layout registers, through the two

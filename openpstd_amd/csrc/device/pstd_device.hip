@@ -293,8 +293,8 @@ __device__ __forceinline__ void gather_finish(const StageArgs &a, const TileDesc
 
 // Forward transform, spectral derivative (the Hermitian filter h, which carries
 // the 1/F of kernel/core/kernel_functions.cpp:243,306), inverse transform.
-// LEAN: the filter is read after the forward transform and twiddles are read
-// per pass from the table `tw` (instead of the preloaded registers `twr`).
+// LEAN: the filter and the twiddles are read from the tables (hsrc, tw) during
+// the transforms instead of the registers preloaded by the caller (h, twr).
 template <int F, int NT, int LAYOUT, bool LEAN>
 __device__ __forceinline__ void derivative_fft(float2 (&v)[8], float2 (&h)[8], const float2 *twr, float2 *buf0,
                                                float2 *buf1, const float2 *hsrc, const float2 *tw) {
@@ -302,16 +302,25 @@ __device__ __forceinline__ void derivative_fft(float2 (&v)[8], float2 (&h)[8], c
     // (p, t) again from the thread index, with F known here
     const int tid = threadIdx.x;
     const int p = LAYOUT == 0 ? tid / T : tid % P, t = LAYOUT == 0 ? tid % T : tid / P;
-    const float2 *w = LEAN ? tw : twr;
-    fft_run<F, P, LAYOUT, false, LEAN>(v, p, t, buf0, buf1, w);
     if constexpr (LEAN) {
+        // filter and twiddles are loaded while the forward transform runs, the
+        // twiddles one pass ahead (fft_run_ahead)
 #pragma unroll
         for (int m = 0; m < 8; ++m) h[m] = hsrc[t + m * (F / 8)];
+        const float2 w0[kTwPass] = {};
+        fft_run_ahead<F, P, LAYOUT, false>(v, p, t, buf0, buf1, tw, w0);
+    } else {
+        fft_run<F, P, LAYOUT, false, false>(v, p, t, buf0, buf1, twr);
     }
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], h[m]);
     __syncthreads();
-    fft_run<F, P, LAYOUT, true, LEAN>(v, p, t, buf0, buf1, w);
+    if constexpr (LEAN) {
+        const float2 w0[kTwPass] = {};
+        fft_run_ahead<F, P, LAYOUT, true>(v, p, t, buf0, buf1, tw, w0);
+    } else {
+        fft_run<F, P, LAYOUT, true, false>(v, p, t, buf0, buf1, twr);
+    }
 }
 
 constexpr int kMaxTw = 28;  // fft_tw_total(F) for F <= 8192
@@ -375,10 +384,11 @@ __device__ __forceinline__ void finish_updates(const StageArgs &a, const TileDes
 // One derivative tile of one layout: NT threads, F = td.F in [FMIN, FMAX].
 // Flavours differ only in when the loads that depend on nothing but the
 // descriptor (filter, twiddles, t_n and PML operands) are issued:
-//  kEarly: before the gather completes (shortest tile latency, most registers);
+//  kEarly: all before the gather completes (most registers);
 //  kLate:  where they are used (fewest live registers, most resident waves);
-//  kMid:   filter and twiddles where used, t_n as soon as the lines arrived,
-//          so its latency hides behind the transforms.
+//  kMid:   filter and twiddles up front (the transforms then never wait on
+//          memory: the vector-memory counter is in order, so a late table
+//          load waits for every load before it), t_n where used.
 // All compute the same values in the same order.
 constexpr int kEarly = 0, kLate = 1, kMid = 2;
 template <int NT, int FMIN, int FMAX, int FL, int LAYOUT>
@@ -389,7 +399,7 @@ __device__ __forceinline__ void tile_layout(const StageArgs &a, const TileDesc &
     GatherRegs R;
     gather_issue<NT, LAYOUT>(a, td, g, p, t, R);
     float2 h[8];
-    constexpr bool LEAN = FL != kEarly;
+    constexpr bool LEAN = FL == kLate;
     float2 twr[LEAN ? 1 : kMaxTw];
     const float2 *tw = a.ctab + td.tw_off;
     const float2 *hsrc = a.ctab + td.h_off;
@@ -414,7 +424,6 @@ __device__ __forceinline__ void tile_layout(const StageArgs &a, const TileDesc &
     float *win = stage + 16 * NT;
     float2 v[8];
     gather_finish<LAYOUT>(a, td, g, p, t, R, stage, win, v);
-    if constexpr (FL == kMid) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
 #define PSTD_FFT_CASE(FV)                                                                                \
     case FV:                                                                                             \
         if constexpr (FMIN <= FV && FMAX >= FV) derivative_fft<FV, NT, LAYOUT, LEAN>(v, h, twr, buf0, buf1, hsrc, tw); \
@@ -430,7 +439,7 @@ __device__ __forceinline__ void tile_layout(const StageArgs &a, const TileDesc &
         default: break;
     }
 #undef PSTD_FFT_CASE
-    if constexpr (FL == kLate) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
+    if constexpr (FL != kEarly) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
     finish_updates(a, td, v, u);
 }
 
