@@ -719,7 +719,8 @@ __global__ void __launch_bounds__(kGemmThreads) stage_usplit_kernel(GemmArgs g, 
 
 // MK: k per LDS chunk (32: 40 KB of LDS, three blocks per CU; 64: 74 KB, two).
 template <int MK>
-__global__ void __launch_bounds__(kGemmThreads) stage_gemm_kernel(GemmArgs g) {
+__global__ void __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(MK == 32 ? 3 : 2)))
+stage_gemm_kernel(GemmArgs g) {
     constexpr int MKP = MK + 8;        // LDS row pitch in halfs
     constexpr int NV = MK / 8 / 2;     // 16-byte vectors per thread and part (128 rows x MK halfs)
     __shared__ __attribute__((aligned(16))) _Float16 uh[kGB * MKP], ul[kGB * MKP], dh[kGB * MKP], dl[kGB * MKP];
@@ -797,33 +798,38 @@ __global__ void __launch_bounds__(kGemmThreads) stage_gemm_kernel(GemmArgs g) {
     const Rsrc rt = block_rsrc(a.tn, td.m_dom, td.m_size);
     const Rsrc rp = block_rsrc(a.pml, td.m_dom, td.m_size);
     const Rsrc ro = block_rsrc(a.out, td.m_dom, td.m_size);
+    // eight values at a time (fewer live registers than a whole 32x32 tile)
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            int off[16];
-            float tnv[16], pm[16];
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = wr * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                const int col = wc * 64 + j * 32 + (lane & 31);
-                const int line = td.dir == 0 ? row : col, n = gd.n0 + (td.dir == 0 ? col : row);
-                const bool ok = line < td.count && n < td.n_out;
-                off[r] = ok ? (td.o_f + (td.line0 + line) * td.o_ls + n * td.o_ps) * 4 : kOOB;
-                tnv[r] = bload(rt, off[r]);
-                pm[r] = use_pml ? bload(rp, off[r]) : 1.f;
-            }
+            for (int hh = 0; hh < 2; ++hh) {
+                int off[8];
+                float tnv[8], pm[8];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = wr * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                const int col = wc * 64 + j * 32 + (lane & 31);
-                const int line = td.dir == 0 ? row : col;
-                const float L = ldexpf(acc[i][j][r], -(uexp[line] + gd.d_exp));
-                const float nv = __builtin_fmaf(-sc, L, tnv[r]);
-                bstore(ro, off[r], nv);
-                if (a.last) bstore(rt, off[r], nv * pm[r]);
+                for (int q = 0; q < 8; ++q) {
+                    const int r = hh * 8 + q;
+                    const int row = wr * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                    const int col = wc * 64 + j * 32 + (lane & 31);
+                    const int line = td.dir == 0 ? row : col, n = gd.n0 + (td.dir == 0 ? col : row);
+                    const bool ok = line < td.count && n < td.n_out;
+                    off[q] = ok ? (td.o_f + (td.line0 + line) * td.o_ls + n * td.o_ps) * 4 : kOOB;
+                    tnv[q] = bload(rt, off[q]);
+                    pm[q] = use_pml ? bload(rp, off[q]) : 1.f;
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int r = hh * 8 + q;
+                    const int row = wr * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                    const int col = wc * 64 + j * 32 + (lane & 31);
+                    const int line = td.dir == 0 ? row : col;
+                    const float L = ldexpf(acc[i][j][r], -(uexp[line] + gd.d_exp));
+                    const float nv = __builtin_fmaf(-sc, L, tnv[q]);
+                    bstore(ro, off[q], nv);
+                    if (a.last) bstore(rt, off[q], nv * pm[q]);
+                }
             }
-        }
 }
 
 // ------------------------------------------------------------- host side
