@@ -127,6 +127,14 @@ __device__ __forceinline__ float bload(Rsrc r, int off_bytes) {
 __device__ __forceinline__ void bstore(Rsrc r, int off_bytes, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off_bytes, 0, 0);
 }
+// two adjacent floats (4-byte aligned suffices for buffer accesses)
+__device__ __forceinline__ float2 bload2(Rsrc r, int off_bytes) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off_bytes, 0, 0));
+}
+__device__ __forceinline__ void bstore2(Rsrc r, int off_bytes, float x, float y) {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, make_float2(x, y)), r, off_bytes, 0, 0);
+}
 
 // Geometry of a derivative tile: F-point lines (run time), NT threads,
 // T = F/8 threads per line pair, P = NT/T pairs; LAYOUT (compile time) 0: X
@@ -186,7 +194,21 @@ __device__ __forceinline__ void gather_issue(const StageArgs &a, const TileDesc 
         off[m][0] = ok && 2 * p < count ? lbase + m * mstep : kOOB;
         off[m][1] = ok && 2 * p + 1 < count ? lbase + lstep + m * mstep : kOOB;
     }
-    if (hasB) {
+    if constexpr (LAYOUT == 1) {
+        // Columns: lines 2p and 2p+1 are adjacent floats (line stride 1), one
+        // 8-byte load fetches both.  An odd last line of a tile reads the float
+        // after its partner (inside the block: every domain block ends with a
+        // spare float); gather_finish drops it.
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const float2 x = bload2(rmA, off[m][0]);
+            R.ra[m][0] = x.x;
+            R.ra[m][1] = x.y;
+            const float2 y = hasB ? bload2(rmB, off[m][0]) : make_float2(0.f, 0.f);
+            R.rb[m][0] = y.x;
+            R.rb[m][1] = y.y;
+        }
+    } else if (hasB) {
 #pragma unroll
         for (int m = 0; m < 8; ++m)
 #pragma unroll
@@ -249,6 +271,7 @@ __device__ __forceinline__ void gather_finish(const StageArgs &a, const TileDesc
     // LDS addresses: line q of the pair at position pos = lb[q] + pos * ps
     const int ps = g.pos_step(), lb[2] = {g.line_base(2 * p), g.line_base(2 * p + 1)};
     const int tps = t * ps, mps = g.T * ps;
+    const bool ok1 = LAYOUT == 0 || 2 * p + 1 < td.count;
     float raw[8][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
@@ -256,7 +279,7 @@ __device__ __forceinline__ void gather_finish(const StageArgs &a, const TileDesc
         const bool mid = pos >= w && pos < w + nm;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            raw[m][q] = R.ra[m][q] + R.rb[m][q];
+            raw[m][q] = q == 0 || ok1 ? R.ra[m][q] + R.rb[m][q] : 0.f;
             if (mid) stage[lb[q] + tps + m * mps] = raw[m][q];
         }
     }
@@ -356,27 +379,51 @@ __device__ __forceinline__ void prefetch_updates(const StageArgs &a, const TileD
         const bool okp = pos >= 0 && pos < n_out;
         u.off[m][0] = okp && ok0 ? base + m * mstep : kOOB;
         u.off[m][1] = okp && ok1 ? base + lstep + m * mstep : kOOB;
+        if constexpr (LAYOUT == 1) {
+            // columns: both lines of the pair in one 8-byte access (as in gather_issue)
+            const float2 x = bload2(rt, u.off[m][0]);
+            const float2 f = use_pml ? bload2(rp, u.off[m][0]) : make_float2(1.f, 1.f);
+            u.tnv[m][0] = x.x;
+            u.tnv[m][1] = x.y;
+            u.pm[m][0] = f.x;
+            u.pm[m][1] = f.y;
+        } else {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            u.tnv[m][q] = bload(rt, u.off[m][q]);
-            u.pm[m][q] = use_pml ? bload(rp, u.off[m][q]) : 1.f;
+            for (int q = 0; q < 2; ++q) {
+                u.tnv[m][q] = bload(rt, u.off[m][q]);
+                u.pm[m][q] = use_pml ? bload(rp, u.off[m][q]) : 1.f;
+            }
         }
     }
 }
 
-__device__ __forceinline__ void finish_updates(const StageArgs &a, const TileDesc &td, const float2 (&v)[8],
+template <int LAYOUT>
+__device__ __forceinline__ void finish_updates(const StageArgs &a, const TileDesc &td, int p, const float2 (&v)[8],
                                                const UpdateRegs &u) {
     const float sc = a.s * td.coef;
     const Rsrc ro = block_rsrc(a.out, td.m_dom, td.m_size);
     const Rsrc rt = block_rsrc(a.tn, td.m_dom, td.m_size);
+    float nv[8][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
+        nv[m][0] = __builtin_fmaf(-sc, v[m].x, u.tnv[m][0]);
+        nv[m][1] = __builtin_fmaf(-sc, v[m].y, u.tnv[m][1]);
+    }
+    if (LAYOUT == 1 && 2 * p + 1 < td.count) {
+        // columns, both lines in the tile: 8-byte stores
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const float L = q ? v[m].y : v[m].x;
-            const float nv = __builtin_fmaf(-sc, L, u.tnv[m][q]);
-            bstore(ro, u.off[m][q], nv);
-            if (a.last) bstore(rt, u.off[m][q], nv * u.pm[m][q]);
+        for (int m = 0; m < 8; ++m) {
+            bstore2(ro, u.off[m][0], nv[m][0], nv[m][1]);
+            if (a.last) bstore2(rt, u.off[m][0], nv[m][0] * u.pm[m][0], nv[m][1] * u.pm[m][1]);
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                bstore(ro, u.off[m][q], nv[m][q]);
+                if (a.last) bstore(rt, u.off[m][q], nv[m][q] * u.pm[m][q]);
+            }
         }
     }
 }
@@ -440,7 +487,7 @@ __device__ __forceinline__ void tile_layout(const StageArgs &a, const TileDesc &
     }
 #undef PSTD_FFT_CASE
     if constexpr (FL != kEarly) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
-    finish_updates(a, td, v, u);
+    finish_updates<LAYOUT>(a, td, p, v, u);
 }
 
 template <int NT, int FMIN, int FMAX, int FL>
@@ -1301,7 +1348,7 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         d.f_px = (int)f;
         f = align(f + cells);
         d.f_py = (int)f;
-        f = align(f + cells);
+        f = align(f + cells + 1);  // a spare float after py (8-byte column accesses)
         d.size = (int)f;
         off += f;
         ctx->doms.push_back(d);
