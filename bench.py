@@ -131,6 +131,8 @@ def main():
                     help="workgroup-to-tile order (default: the autotuner's choice)")
     ap.add_argument("--flavour", default="auto", choices=["auto", "early", "late", "mid"],
                     help="stage-kernel flavour (default: the autotuner's choice)")
+    ap.add_argument("--tile-threads", default="auto", choices=["auto", "256", "512"],
+                    help="block size for lines of FFT length <= 2048 (default: the autotuner's choice)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a copy of MIB MiB (for scaling HBM byte counters)")
@@ -162,6 +164,8 @@ def main():
         dev.set_tile_order(args.tile_order == "xcd")
     if args.flavour != "auto":
         dev.set_kernel_flavour(args.flavour)
+    if args.tile_threads != "auto":
+        dev.set_tile_threads(int(args.tile_threads))
     tuning = dev.tuning()
     cells = dev.cells_per_stage
 
@@ -207,7 +211,9 @@ def main():
         "config": {"workload": desc, "scenes": len(confs), "cells_per_stage": cells, "user_cells": user,
                    "domains": len(infos), "rk_stages_per_step": 6, "graph_steps": args.graph_steps,
                    "launch_shape": {"flavour": ["early", "late", "mid"][tuning["lean"]], "xcd_order": tuning["xcd_order"],
-                                    "autotuned": args.tile_order == "auto" and args.flavour == "auto"},
+                                    "tile_threads": tuning["tile_threads"],
+                                    "autotuned": args.tile_order == "auto" and args.flavour == "auto"
+                                    and args.tile_threads == "auto"},
                    "path": ("FP32 FFT (fused LDS line derivative + RK update)" if args.form == "fft" else
                             "dense derivative matrices on MFMA, FP16 hi+lo split operands, FP32 accumulate "
                             "(short segments in the FP32 FFT form)")},

@@ -214,10 +214,16 @@ int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps);
  *    twiddles, t_n) before the transforms (short tiles, latency-bound scenes),
  *    1 loads them where they are used (fewer registers, more resident waves,
  *    throughput-bound scenes), 2 loads filter and twiddles up front and t_n
- *    where used.
- * pstd_ctx_get_tuning reports the choice and its timed GPU ms per stage. */
+ *    where used;
+ *  - tile threads: block size of the kernel for lines of FFT length <= 2048,
+ *    256 or 512 (512 puts twice the lines in a tile: for columns, twice the
+ *    contiguous bytes of each row per wave access).
+ * pstd_ctx_get_tuning reports the choice and its timed GPU ms per stage,
+ * pstd_ctx_get_tile_threads the block size. */
 int pstd_ctx_set_tile_order(pstd_ctx *ctx, int xcd_contiguous);
 int pstd_ctx_set_kernel_flavour(pstd_ctx *ctx, int flavour);
+int pstd_ctx_set_tile_threads(pstd_ctx *ctx, int threads);
+int pstd_ctx_get_tile_threads(const pstd_ctx *ctx);
 int pstd_ctx_get_tuning(const pstd_ctx *ctx, int32_t *flavour, int32_t *xcd_contiguous, double *stage_ms);
 /* Work of the GEMM-form segments in one RK stage: algorithmic MFMA flops
  * (3 split products x 2 x K x outputs x lines) and line outputs. */

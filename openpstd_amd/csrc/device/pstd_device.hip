@@ -43,7 +43,6 @@ using namespace pstdamd::dev;
 
 namespace {
 
-constexpr int kSmallThreads = 256;  // block size of the F <= 2048 class
 constexpr int kPassChunk = 2048;    // elements per passthrough tile
 constexpr int kMaxDomainFloats = 1 << 29;  // one domain block (4 fields) per buffer resource
 
@@ -509,8 +508,9 @@ __device__ void sample_receivers(const StageArgs &a) {
 // Fields no segment updates keep their t_n value (openPSTD leaves their L at 0,
 // kernel/Solver.cpp:174-181); the last stage applies the PML factor.  Each
 // thread issues all of its loads before its stores.
+template <int NT>
 __device__ void passthrough_tile(const StageArgs &a, const TileDesc &td) {
-    constexpr int PER = kPassChunk / kSmallThreads;
+    constexpr int PER = kPassChunk / NT;
     const Rsrc rt = block_rsrc(a.tn, td.m_dom, td.m_size);
     const Rsrc rp = block_rsrc(a.pml, td.m_dom, td.m_size);
     const Rsrc ro = block_rsrc(a.out, td.m_dom, td.m_size);
@@ -519,7 +519,7 @@ __device__ void passthrough_tile(const StageArgs &a, const TileDesc &td) {
     int off[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const int e = threadIdx.x + k * kSmallThreads;
+        const int e = threadIdx.x + k * NT;
         off[k] = e < td.count ? (td.o_f + td.line0 + e) * 4 : kOOB;
         v[k] = bload(rt, off[k]);
         f[k] = use_pml ? bload(rp, off[k]) : 1.f;
@@ -543,12 +543,13 @@ __device__ __forceinline__ int tile_index(int b, int n, int remap) {
     return x * q + min(x, r) + l;
 }
 
-// two LDS buffers of P * F = 8 * NT complex values each (padded)
-constexpr int kSmallBuf = lds_padded(8 * kSmallThreads);
-
-template <int FL>
-__global__ void __launch_bounds__(kSmallThreads) stage_small_kernel(StageArgs a) {
-    __shared__ float2 lds[2 * kSmallBuf];
+// F <= 2048 class, NT = 256 or 512 threads: two LDS buffers of P * F = 8 * NT
+// complex values each (padded).  Wider blocks hold more lines per tile (for
+// columns, more of each row per wave access) at the same waves per CU.
+template <int NT, int FL>
+__global__ void __launch_bounds__(NT) stage_small_kernel(StageArgs a) {
+    constexpr int kBuf = lds_padded(8 * NT);
+    __shared__ float2 lds[2 * kBuf];
     // a copy taken before any store: the descriptor stays in scalar registers
     const TileDesc td = a.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
     if (td.kind == 2) {
@@ -556,10 +557,10 @@ __global__ void __launch_bounds__(kSmallThreads) stage_small_kernel(StageArgs a)
         return;
     }
     if (td.kind == 1) {
-        passthrough_tile(a, td);
+        passthrough_tile<NT>(a, td);
         return;
     }
-    derivative_tile<kSmallThreads, 128, 2048, FL>(a, td, lds, lds + kSmallBuf);
+    derivative_tile<NT, 128, 2048, FL>(a, td, lds, lds + kBuf);
 }
 
 template <int F, int FL>
@@ -907,6 +908,11 @@ struct pstd_ctx {
     std::vector<DomainDev> doms;
     std::vector<pstd_plan_domain> plan_domains;
     std::vector<TileDesc> tiles_small, tiles_4096, tiles_8192;
+    // FFT-form segments (line0/count = the segment's line range) and the
+    // passthrough/receiver tiles: the tile lists are cut from them for the
+    // block size of the F <= 2048 class (small_nt, autotuned: 256 or 512)
+    std::vector<TileDesc> fft_segs, tiles_extra, tiles_all;
+    int small_nt = 256;
     std::vector<GemmDesc> tiles_gemm, tiles_split;
     std::vector<SplitPiece> pieces;
     long long u_halfs = 0, u_exps = 0;  // split-input scratch of the GEMM form
@@ -1044,9 +1050,15 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
     const int fl = ctx->lean;
     if (ns) {
         a.tiles = ctx->d_tiles;
-        if (fl == kLate) hipLaunchKernelGGL(stage_small_kernel<kLate>, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
-        else if (fl == kMid) hipLaunchKernelGGL(stage_small_kernel<kMid>, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
-        else hipLaunchKernelGGL(stage_small_kernel<kEarly>, dim3(ns), dim3(kSmallThreads), 0, ctx->stream, a);
+        if (ctx->small_nt == 512) {
+            if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<512, kLate>), dim3(ns), dim3(512), 0, ctx->stream, a);
+            else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<512, kMid>), dim3(ns), dim3(512), 0, ctx->stream, a);
+            else hipLaunchKernelGGL((stage_small_kernel<512, kEarly>), dim3(ns), dim3(512), 0, ctx->stream, a);
+        } else {
+            if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<256, kLate>), dim3(ns), dim3(256), 0, ctx->stream, a);
+            else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<256, kMid>), dim3(ns), dim3(256), 0, ctx->stream, a);
+            else hipLaunchKernelGGL((stage_small_kernel<256, kEarly>), dim3(ns), dim3(256), 0, ctx->stream, a);
+        }
     }
     if (n4) {
         a.tiles = ctx->d_tiles + ns;
@@ -1189,6 +1201,40 @@ static int enqueue_steps(pstd_ctx *ctx, int chunk) {
 // candidate on the freshly uploaded (all-zero) state and keeps the fastest.
 // Zero fields stay exactly zero under the update, and the arenas are cleared
 // again afterwards, so tuning leaves no trace in the results.
+// Tile lists for the current block size of the F <= 2048 class: a segment's
+// lines in tiles of NL = 2 P lines (P = NT / (F/8) pairs; one pair per tile
+// above 2048), then the passthrough/receiver tiles; uploaded as
+// small | 4096 | 8192.  The pairs (line0 + 2p, line0 + 2p + 1) are the same
+// for every NT, so results do not depend on it.
+static int upload_tiles(pstd_ctx *ctx) {
+    ctx->tiles_small.clear();
+    ctx->tiles_4096.clear();
+    ctx->tiles_8192.clear();
+    for (const TileDesc &g : ctx->fft_segs) {
+        const int per = 2 * (g.F >= 4096 ? 1 : ctx->small_nt / (g.F / 8));
+        for (int l = 0; l < g.count; l += per) {
+            TileDesc t = g;
+            t.line0 = g.line0 + l;
+            t.count = std::min(per, g.count - l);
+            if (g.F == 4096) ctx->tiles_4096.push_back(t);
+            else if (g.F == 8192) ctx->tiles_8192.push_back(t);
+            else ctx->tiles_small.push_back(t);
+        }
+    }
+    ctx->tiles_small.insert(ctx->tiles_small.end(), ctx->tiles_extra.begin(), ctx->tiles_extra.end());
+    ctx->tiles_all = ctx->tiles_small;
+    ctx->tiles_all.insert(ctx->tiles_all.end(), ctx->tiles_4096.begin(), ctx->tiles_4096.end());
+    ctx->tiles_all.insert(ctx->tiles_all.end(), ctx->tiles_8192.begin(), ctx->tiles_8192.end());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_tiles) HIP_TRY(hipFree(ctx->d_tiles));
+    ctx->d_tiles = nullptr;
+    HIP_TRY(hipMalloc(&ctx->d_tiles, std::max<size_t>(1, ctx->tiles_all.size()) * sizeof(TileDesc)));
+    if (!ctx->tiles_all.empty())
+        HIP_TRY(hipMemcpyAsync(ctx->d_tiles, ctx->tiles_all.data(), ctx->tiles_all.size() * sizeof(TileDesc),
+                               hipMemcpyHostToDevice, ctx->stream));
+    return PSTD_OK;
+}
+
 static int autotune(pstd_ctx *ctx) {
     static const int cand[6][2] = {{kLate, 1}, {kMid, 1}, {kEarly, 1}, {kLate, 0}, {kMid, 0}, {kEarly, 0}};  // (flavour, xcd order)
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1205,20 +1251,34 @@ static int autotune(pstd_ctx *ctx) {
         return rc;
     };
     float best = 0;
-    int bi = 0, rc = PSTD_OK;
-    for (int c = 0; c < 6 && rc == PSTD_OK; ++c) {
-        ctx->lean = cand[c][0];
-        ctx->xcd_remap = cand[c][1];
-        float ms = 0;
-        rc = time_steps(ms);
-        if (rc == PSTD_OK && (c == 0 || ms < best)) {
-            best = ms;
-            bi = c;
+    int bi = 0, bnt = 256, rc = PSTD_OK;
+    bool has_small = false;
+    for (const TileDesc &g : ctx->fft_segs) has_small = has_small || g.F <= 2048;
+    for (int nt = 256; nt <= 512 && rc == PSTD_OK; nt *= 2) {
+        if (nt != ctx->small_nt) {
+            if (!has_small) break;  // the block size only shapes the F <= 2048 class
+            ctx->small_nt = nt;
+            rc = upload_tiles(ctx);
+        }
+        for (int c = 0; c < 6 && rc == PSTD_OK; ++c) {
+            ctx->lean = cand[c][0];
+            ctx->xcd_remap = cand[c][1];
+            float ms = 0;
+            rc = time_steps(ms);
+            if (rc == PSTD_OK && ((nt == 256 && c == 0) || ms < best)) {
+                best = ms;
+                bi = c;
+                bnt = nt;
+            }
         }
     }
     if (rc == PSTD_OK) {
         ctx->lean = cand[bi][0];
         ctx->xcd_remap = cand[bi][1];
+        if (bnt != ctx->small_nt) {
+            ctx->small_nt = bnt;
+            rc = upload_tiles(ctx);
+        }
     }
     if (rc == PSTD_OK && !ctx->tiles_gemm.empty()) {  // GEMM chunk depth, with the chosen flavour and order
         ctx->gemm_mk = 32;
@@ -1311,6 +1371,8 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     ctx->release();
     ctx->doms.clear();
     ctx->tiles_small.clear();
+    ctx->fft_segs.clear();
+    ctx->tiles_extra.clear();
     ctx->tiles_4096.clear();
     ctx->tiles_8192.clear();
     ctx->tiles_gemm.clear();
@@ -1509,17 +1571,9 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
             if (rc) return rc;
             continue;
         }
-        // tiles: NL = 2 * P lines per block
-        const int P = g.F >= 4096 ? 1 : kSmallThreads / (g.F / 8);
-        const int per = 2 * P;
-        for (int l = s.line_begin; l < s.line_end; l += per) {
-            TileDesc t = g;
-            t.line0 = l;
-            t.count = std::min(per, s.line_end - l);
-            if (g.F == 4096) ctx->tiles_4096.push_back(t);
-            else if (g.F == 8192) ctx->tiles_8192.push_back(t);
-            else ctx->tiles_small.push_back(t);
-        }
+        g.line0 = s.line_begin;
+        g.count = s.line_end - s.line_begin;
+        ctx->fft_segs.push_back(g);
     }
     // passthrough tiles for fields no segment updates (rigid domains, directions
     // openPSTD does not update).  covered[] index: 0 px, 1 py, 2 vx, 3 vy
@@ -1538,14 +1592,14 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
                 t.m_size = d.size;
                 t.o_f = offs[f];
                 t.is_pml = d.is_pml;
-                ctx->tiles_small.push_back(t);
+                ctx->tiles_extra.push_back(t);
             }
         }
     }
     if (plan->n_receivers > 0) {
         TileDesc t{};
         t.kind = 2;  // receiver sampling of the previous step (first stage only)
-        ctx->tiles_small.push_back(t);
+        ctx->tiles_extra.push_back(t);
     }
     if (ctab.empty()) ctab.push_back(make_float2(0, 0));
     if (ftab.empty()) ftab.push_back(0);
@@ -1593,13 +1647,10 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     HIP_TRY(hipMemcpyAsync(ctx->ctab, ctab.data(), ctab.size() * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMalloc(&ctx->ftab, ftab.size() * sizeof(float)));
     HIP_TRY(hipMemcpyAsync(ctx->ftab, ftab.data(), ftab.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    std::vector<TileDesc> all = ctx->tiles_small;
-    all.insert(all.end(), ctx->tiles_4096.begin(), ctx->tiles_4096.end());
-    all.insert(all.end(), ctx->tiles_8192.begin(), ctx->tiles_8192.end());
-    HIP_TRY(hipMalloc(&ctx->d_tiles, std::max<size_t>(1, all.size()) * sizeof(TileDesc)));
-    if (!all.empty())
-        HIP_TRY(hipMemcpyAsync(ctx->d_tiles, all.data(), all.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
-                               ctx->stream));
+    {
+        const int rc = upload_tiles(ctx);
+        if (rc) return rc;
+    }
     if (!ctx->tiles_gemm.empty()) {
         HIP_TRY(hipMalloc(&ctx->d_gemm, ctx->tiles_gemm.size() * sizeof(GemmDesc)));
         HIP_TRY(hipMemcpyAsync(ctx->d_gemm, ctx->tiles_gemm.data(), ctx->tiles_gemm.size() * sizeof(GemmDesc),
@@ -1859,6 +1910,23 @@ int pstd_ctx_get_tuning(const pstd_ctx *ctx, int32_t *lean, int32_t *xcd_contigu
     if (xcd_contiguous) *xcd_contiguous = ctx->xcd_remap;
     if (stage_ms) *stage_ms = ctx->tuned_stage_ms;
     return PSTD_OK;
+}
+
+int pstd_ctx_set_tile_threads(pstd_ctx *ctx, int threads) {
+    if (!ctx || !ctx->planned || (threads != 256 && threads != 512)) return PSTD_ERR_INVALID;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graph();
+    ctx->small_nt = threads;
+    const int rc = upload_tiles(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PSTD_OK;
+}
+
+int pstd_ctx_get_tile_threads(const pstd_ctx *ctx) {
+    if (!ctx || !ctx->planned) return PSTD_ERR_INVALID;
+    return ctx->small_nt;
 }
 
 int pstd_ctx_set_tile_order(pstd_ctx *ctx, int xcd_contiguous) {

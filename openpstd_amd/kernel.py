@@ -239,12 +239,19 @@ class _Context:
         f = self.FLAVOURS.get(flavour, flavour)
         _check(lib().pstd_ctx_set_kernel_flavour(self.ctx, int(f)), self._err)
 
+    def set_tile_threads(self, threads):
+        """Block size (256 or 512) of the stage kernel for lines of FFT length <= 2048."""
+        _check(lib().pstd_ctx_set_tile_threads(self.ctx, int(threads)), self._err)
+
     def tuning(self):
-        """The autotuner's choice: {"lean" (flavour 0 early, 1 late, 2 mid), "xcd_order", "stage_ms"}."""
+        """The autotuner's choice: {"lean" (flavour 0 early, 1 late, 2 mid), "xcd_order", "tile_threads",
+        "stage_ms"}."""
         lean, xcd = C.c_int32(), C.c_int32()
         ms = C.c_double()
         _check(lib().pstd_ctx_get_tuning(self.ctx, C.byref(lean), C.byref(xcd), C.byref(ms)), self._err)
-        return {"lean": lean.value, "xcd_order": xcd.value, "stage_ms": ms.value}
+        nt = lib().pstd_ctx_get_tile_threads(self.ctx)
+        _check(min(nt, 0), self._err)
+        return {"lean": lean.value, "xcd_order": xcd.value, "tile_threads": nt, "stage_ms": ms.value}
 
     def _get(self, i, name, shape):
         out = np.zeros(shape, dtype=np.float32)

@@ -89,17 +89,20 @@ def test_graph_replay_is_bitwise_eager(graph_steps):
     assert np.array_equal(eager.field(1, "vx0"), g.field(1, "vx0"))
 
 
-@pytest.mark.parametrize("shape", [(0, 1), (1, 0), (1, 1), (2, 0), (2, 1)])
+@pytest.mark.parametrize("shape", [(0, 1, 256), (1, 0, 256), (1, 1, 256), (2, 0, 256), (2, 1, 256), (0, 0, 512),
+                                   (1, 1, 512), (2, 1, 512)])
 def test_launch_shape_is_bitwise_neutral(shape):
-    """Every (kernel flavour, tile order) the autotuner may pick gives the same bits."""
+    """Every (kernel flavour, tile order, tile threads) the autotuner may pick gives the same bits."""
     conf = W.lshape(2, 25)
     k, a = single(conf)
     _, b = single(conf)
     print("autotuned:", a.tuning())
     a.set_kernel_flavour(0)
     a.set_tile_order(False)
+    a.set_tile_threads(256)
     b.set_kernel_flavour(shape[0])
     b.set_tile_order(shape[1])
+    b.set_tile_threads(shape[2])
     assert np.array_equal(a.advance(25), b.advance(25))
     for d in range(k.domain_count()):
         for f in ("p0", "vx0", "py0"):
@@ -120,3 +123,24 @@ def test_long_lines_lean_flavour_bitwise():
     b.advance(3)
     for d in range(k.domain_count()):
         assert np.array_equal(a.field(d, "p0"), b.field(d, "p0")), d
+
+
+@pytest.mark.parametrize("nx, ny", [(301, 1000), (1000, 301), (77, 129)])
+def test_tile_threads_bitwise(nx, ny):
+    """Block sizes 256 and 512 of the F <= 2048 class give the same bits: the
+    line pairs are the same, only the number of pairs per tile differs (odd
+    line counts leave a lone last line in both)."""
+    conf = P.PSTDConfiguration()
+    W.base_settings(conf, 4)
+    conf.Domains = [W.domain(0, 0, nx, ny, 1.0)]
+    conf.Speakers = [(W._world_pos(nx / 2), W._world_pos(ny / 2), 0.0)]
+    k, a = single(conf)
+    _, b = single(conf)
+    a.set_tile_threads(256)
+    b.set_tile_threads(512)
+    assert b.tuning()["tile_threads"] == 512
+    a.advance(4)
+    b.advance(4)
+    for d in range(k.domain_count()):
+        for f in ("p0", "vx0", "vy0"):
+            assert np.array_equal(a.field(d, f), b.field(d, f)), (d, f)
