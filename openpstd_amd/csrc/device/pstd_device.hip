@@ -169,6 +169,7 @@ struct GatherRegs {
     float ra[8][2], rb[8][2];
     float h1a[4], h1b[4], h2a[4], h2b[4];
     int hpos1[4], hpos2[4];
+    float win;  // window coefficient threadIdx.x (threads 0..2w)
 };
 
 template <int NT, int LAYOUT>
@@ -176,6 +177,8 @@ __device__ __forceinline__ void gather_issue(const StageArgs &a, const TileDesc 
                                              int t, GatherRegs &R) {
     const int w = td.wlen, nm = td.n_main, vel = td.vel, count = td.count;
     __builtin_assume(w > 0 && w <= 32);
+    // the window, with the lines (not after them: one memory latency less)
+    R.win = (int)threadIdx.x <= 2 * w ? a.ftab[td.win_off + threadIdx.x] : 0.f;
     const float *A = vel ? a.vsrc : a.p0a;
     const bool hasB = !vel && a.p0b != nullptr;
     const float *B = hasB ? a.p0b : A;
@@ -289,7 +292,7 @@ __device__ __forceinline__ void gather_finish(const StageArgs &a, const TileDesc
             stage[R.hpos2[k]] = R.h2a[k] + R.h2b[k];
         }
     }
-    if ((int)threadIdx.x <= 2 * w) win[threadIdx.x] = a.ftab[td.win_off + threadIdx.x];
+    if ((int)threadIdx.x <= 2 * w) win[threadIdx.x] = R.win;
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
