@@ -19,6 +19,10 @@ same counters' reading for a copy of known size), read from the committed
 profile summary of the workload when there is one.
 cpu_baseline: the CPU oracle (FP32 restatement of the reference solver with
 OpenMP over calc tasks) on this host, over a bounded sample of the same scene.
+--plain-compare N (batches): the first N scenes of the batch also run one at a
+time, each on its own solver in the FP32 FFT form (the plain path), and
+"plain_sequential" reports their cell-RK-stage/s (GPU time of the timed stage
+launches only, host gaps between scenes excluded) and batched / plain.
 """
 import argparse
 import json
@@ -60,6 +64,27 @@ def workload(name, steps):
                                          "cells, edge absorption U[0,1], 1 source + 1 receiver each, advanced "
                                          "together (one launch per RK stage), FP32 FFT path")
     raise SystemExit(f"unknown workload {name}")
+
+
+def plain_sequential(confs, n, steps):
+    """The first n scenes advanced one at a time (one solver each, FP32 FFT
+    form, autotuned like the batch): summed cell-stages over summed GPU time of
+    their timed stage launches."""
+    import openpstd_amd as P
+    gpu_s, cell_stages = 0.0, 0
+    for conf in confs[:n]:
+        k = P.PSTDKernel(GPU=True)
+        k.initialize_kernel(conf)
+        k.set_form("fft")
+        dev = k.prepare()
+        dev.advance_async(2)
+        dev.synchronize()
+        ms, launches = dev.profile_stages(steps)
+        gpu_s += ms * 1e-3 * launches
+        cell_stages += dev.cells_per_stage * launches
+        del dev, k
+    return {"value": cell_stages / gpu_s, "unit": UNIT, "scenes": min(n, len(confs)), "steps_per_scene": steps,
+            "form": "fft"}
 
 
 def cpu_baseline(confs, seconds=10.0, max_steps=400):
@@ -134,6 +159,8 @@ def main():
     ap.add_argument("--tile-threads", default="auto", choices=["auto", "256", "512"],
                     help="block size for lines of FFT length <= 2048 (default: the autotuner's choice)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-compare", type=int, default=0, metavar="N",
+                    help="batches: also time the first N scenes one at a time in the plain FFT form")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a copy of MIB MiB (for scaling HBM byte counters)")
     args = ap.parse_args()
@@ -219,6 +246,10 @@ def main():
                             "(short segments in the FP32 FFT form)")},
         "roofline": roofline,
     }
+    if args.plain_compare and len(confs) > 1:
+        plain = plain_sequential(confs, args.plain_compare, min(steps, 20))
+        plain["batched_over_plain"] = value / plain["value"]
+        out["plain_sequential"] = plain
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(confs)
     print(json.dumps(out))
