@@ -7,9 +7,12 @@
 2. Long-run oracle series (FP64 CPU oracle), too slow to recompute in a test:
      room_receivers.npz, lshape_receivers.npz, square1024_receivers.npz
      (a 1024 x 1024-cell domain whose lines fill the GEMM form's blocks, 300
-     steps, 16 receivers within 120 cells of the source)
+     steps, 16 receivers within 120 cells of the source),
+     sources1024_receivers.npz (the 256-source geometry: 1024 x 1024 cells as
+     two 512 x 1024 domains, its first two seeded source positions with 8
+     seeded receivers each, the full 3000 steps, FP64)
 
-    python tests/golden/make_fixtures.py [--kat] [--series] [--square]
+    python tests/golden/make_fixtures.py [--kat] [--series] [--square] [--sources]
 """
 import json
 import os
@@ -69,6 +72,17 @@ def series(which=("room", "lshape")):
         np.savez(os.path.join(HERE, f"{name}_receivers.npz"), **out)
 
 
+def sources(n_sources=2, steps=3000):
+    from oracle import binding as ob
+    from openpstd_amd import workloads as W
+    threads = int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1))
+    out = {}
+    for i, conf in enumerate(W.source_batch(0, steps, n_sources=n_sources)):
+        out[f"fp64_source{i}"] = ob.OracleScene(conf, 64, threads=threads).step(steps)
+        print("source", i, "done", flush=True)
+    np.savez(os.path.join(HERE, "sources1024_receivers.npz"), **out)
+
+
 if __name__ == "__main__":
     args = sys.argv[1:] or ["--kat", "--series"]
     if "--kat" in args:
@@ -77,3 +91,5 @@ if __name__ == "__main__":
         series()
     if "--square" in args:
         series(("square1024",))
+    if "--sources" in args:
+        sources()
