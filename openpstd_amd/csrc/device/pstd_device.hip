@@ -229,36 +229,78 @@ __device__ __forceinline__ void gather_issue(const StageArgs &a, const TileDesc 
     }
     // halos: item i of a side is (line ll, offset j), j in slots of 2^wlog >= w;
     // rows: j fastest, columns: ll fastest (coalesced either way)
-    const int n_items = (2 * g.P) << td.wlog;
-    const int nk = (n_items + NT - 1) / NT;
     const Rsrc r1A = block_rsrc(A, td.s1_dom, td.s1_size);
     const Rsrc r1B = block_rsrc(B, td.s1_dom + (td.s1_b - td.s1_a), td.s1_size - (td.s1_b - td.s1_a));
     const Rsrc r2A = block_rsrc(A, td.s2_dom, td.s2_size);
     const Rsrc r2B = block_rsrc(B, td.s2_dom + (td.s2_b - td.s2_a), td.s2_size - (td.s2_b - td.s2_a));
+    if constexpr (LAYOUT == 1) {
+        // columns: item i is (pair pp, offset j), both lines of the pair (adjacent
+        // floats) in one 8-byte load into registers 2k, 2k+1 (at most 2 items per
+        // thread: P 2^wlog <= 2 NT); gather_finish drops a lone last line's value
+        const int n_items = g.P << td.wlog;
+        const int nk = (n_items + NT - 1) / NT;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        R.hpos1[k] = R.hpos2[k] = -1;
-        R.h1a[k] = R.h1b[k] = R.h2a[k] = R.h2b[k] = 0.f;
-        if (k < nk) {
-            const int i = threadIdx.x + k * NT;
-            const int ll = LAYOUT == 0 ? i >> td.wlog : i & (2 * g.P - 1);
-            const int j = LAYOUT == 0 ? i & ((1 << td.wlog) - 1) : i >> (g.logP + 1);
-            const bool item = i < n_items && j < w;
-            const bool ok = item && ll < count;
-            const int line = td.line0 + ll;
-            const int o1 = ok && td.s1_len >= 0
-                               ? (td.s1_a + (line + td.s1_line) * td.s1_ls + (td.s1_len - w - vel + j) * td.s1_ps) * 4
-                               : kOOB;
-            const int o2 = ok && td.s2_len >= 0 ? (td.s2_a + (line + td.s2_line) * td.s2_ls + (j + vel) * td.s2_ps) * 4
-                                                : kOOB;
-            R.h1a[k] = bload(r1A, o1);
-            R.h2a[k] = bload(r2A, o2);
-            if (hasB) {
-                R.h1b[k] = bload(r1B, o1);
-                R.h2b[k] = bload(r2B, o2);
+        for (int k = 0; k < 4; ++k) {
+            R.hpos1[k] = R.hpos2[k] = -1;
+            R.h1a[k] = R.h1b[k] = R.h2a[k] = R.h2b[k] = 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (k < nk) {
+                const int i = threadIdx.x + k * NT;
+                const int pp = i & (g.P - 1), j = i >> g.logP;
+                const bool item = i < n_items && j < w;
+                const bool ok = item && 2 * pp < count;
+                const int line = td.line0 + 2 * pp;
+                const int o1 = ok && td.s1_len >= 0
+                                   ? (td.s1_a + (line + td.s1_line) * td.s1_ls + (td.s1_len - w - vel + j) * td.s1_ps) * 4
+                                   : kOOB;
+                const int o2 = ok && td.s2_len >= 0 ? (td.s2_a + (line + td.s2_line) * td.s2_ls + (j + vel) * td.s2_ps) * 4
+                                                    : kOOB;
+                const float2 x1 = bload2(r1A, o1), x2 = bload2(r2A, o2);
+                R.h1a[2 * k] = x1.x;
+                R.h1a[2 * k + 1] = x1.y;
+                R.h2a[2 * k] = x2.x;
+                R.h2a[2 * k + 1] = x2.y;
+                if (hasB) {
+                    const float2 y1 = bload2(r1B, o1), y2 = bload2(r2B, o2);
+                    R.h1b[2 * k] = y1.x;
+                    R.h1b[2 * k + 1] = y1.y;
+                    R.h2b[2 * k] = y2.x;
+                    R.h2b[2 * k + 1] = y2.y;
+                }
+                R.hpos1[k] = item ? g.stage_at(2 * pp, j) : -1;
+                R.hpos2[k] = item ? g.stage_at(2 * pp, w + nm + j) : -1;
             }
-            R.hpos1[k] = item ? g.stage_at(ll, j) : -1;
-            R.hpos2[k] = item ? g.stage_at(ll, w + nm + j) : -1;
+        }
+    } else {
+        const int n_items = (2 * g.P) << td.wlog;
+        const int nk = (n_items + NT - 1) / NT;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            R.hpos1[k] = R.hpos2[k] = -1;
+            R.h1a[k] = R.h1b[k] = R.h2a[k] = R.h2b[k] = 0.f;
+            if (k < nk) {
+                const int i = threadIdx.x + k * NT;
+                const int ll = i >> td.wlog;
+                const int j = i & ((1 << td.wlog) - 1);
+                const bool item = i < n_items && j < w;
+                const bool ok = item && ll < count;
+                const int line = td.line0 + ll;
+                const int o1 = ok && td.s1_len >= 0
+                                   ? (td.s1_a + (line + td.s1_line) * td.s1_ls + (td.s1_len - w - vel + j) * td.s1_ps) * 4
+                                   : kOOB;
+                const int o2 = ok && td.s2_len >= 0 ? (td.s2_a + (line + td.s2_line) * td.s2_ls + (j + vel) * td.s2_ps) * 4
+                                                    : kOOB;
+                R.h1a[k] = bload(r1A, o1);
+                R.h2a[k] = bload(r2A, o2);
+                if (hasB) {
+                    R.h1b[k] = bload(r1B, o1);
+                    R.h2b[k] = bload(r2B, o2);
+                }
+                R.hpos1[k] = item ? g.stage_at(ll, j) : -1;
+                R.hpos2[k] = item ? g.stage_at(ll, w + nm + j) : -1;
+            }
         }
     }
 }
@@ -285,11 +327,25 @@ __device__ __forceinline__ void gather_finish(const StageArgs &a, const TileDesc
             if (mid) stage[lb[q] + tps + m * mps] = raw[m][q];
         }
     }
+    if constexpr (LAYOUT == 1) {
+        // a pair's two halo values are adjacent in the staging (stage_at(2pp, j) even)
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (R.hpos1[k] >= 0) {
-            stage[R.hpos1[k]] = R.h1a[k] + R.h1b[k];
-            stage[R.hpos2[k]] = R.h2a[k] + R.h2b[k];
+        for (int k = 0; k < 2; ++k) {
+            if (R.hpos1[k] >= 0) {
+                const bool two = (R.hpos1[k] & (2 * g.P - 1)) + 1 < td.count;
+                *reinterpret_cast<float2 *>(&stage[R.hpos1[k]]) =
+                    make_float2(R.h1a[2 * k] + R.h1b[2 * k], two ? R.h1a[2 * k + 1] + R.h1b[2 * k + 1] : 0.f);
+                *reinterpret_cast<float2 *>(&stage[R.hpos2[k]]) =
+                    make_float2(R.h2a[2 * k] + R.h2b[2 * k], two ? R.h2a[2 * k + 1] + R.h2b[2 * k + 1] : 0.f);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (R.hpos1[k] >= 0) {
+                stage[R.hpos1[k]] = R.h1a[k] + R.h1b[k];
+                stage[R.hpos2[k]] = R.h2a[k] + R.h2b[k];
+            }
         }
     }
     if ((int)threadIdx.x <= 2 * w) win[threadIdx.x] = R.win;

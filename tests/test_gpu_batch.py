@@ -139,6 +139,9 @@ def test_tile_threads_bitwise(nx, ny):
     a.set_tile_threads(256)
     b.set_tile_threads(512)
     assert b.tuning()["tile_threads"] == 512
+    with pytest.raises(P.PSTDError):
+        b.set_tile_threads(384)  # only 256 and 512 exist
+    assert b.tuning()["tile_threads"] == 512
     a.advance(4)
     b.advance(4)
     for d in range(k.domain_count()):

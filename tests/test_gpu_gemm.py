@@ -131,3 +131,28 @@ def test_square1024_forms_vs_golden():
           f"FFT {errs['fft'][0]:.2e} / {errs['fft'][1]:.2e}, CPU FP32 oracle {e32:.2e}")
     assert errs["gemm"][0] <= 1e-4 and errs["gemm"][1] <= 5e-6
     assert errs["fft"][0] <= 1e-4
+
+
+def test_sources1024_per_source_vs_golden():
+    """The 256-source geometry (1024 x 1024 cells as two 512 x 1024 domains, PML
+    around): its first two seeded source positions advanced as one batch for
+    the full 3000 steps, 8 seeded receivers each, against the committed FP64
+    oracle series, error stated per source; both forms within the 1e-4 bar."""
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sources1024_receivers.npz"))
+    steps = 3000
+    confs = W.source_batch(0, steps, n_sources=2)
+    errs = {}
+    for form in ("gemm", "fft"):
+        b = P.BatchSolver(confs, form=form)
+        if form == "gemm":
+            assert b.gemm_work()[0] > 0
+        got = b.advance(steps)
+        for i in range(len(confs)):
+            s, want = b.scene_samples(got, i), g[f"fp64_source{i}"]
+            assert np.abs(want).max() > 1e-3
+            errs[(form, i)] = (rel(s, want), np.abs(s - want).max())
+    for (form, i), (e, m) in sorted(errs.items()):
+        print(f"source {i}, {form} form, 3000 steps: receiver rel-L2 vs FP64 {e:.2e}, max-abs {m:.2e}")
+    for (form, i), (e, m) in errs.items():
+        assert e <= 1e-4, (form, i, e)
+        assert m <= 5e-6, (form, i, m)
