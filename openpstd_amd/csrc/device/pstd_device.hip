@@ -64,6 +64,7 @@ struct DomainDev {
 //  kind 1: passthrough — elements [line0, line0+count) of one field that no
 //          segment updates (rigid domains, directions openPSTD skips).
 //  kind 2: receiver sampling (first stage of a step).
+//  kind 3: the segment addressing of a GEMM-form line block (GemmDesc::t).
 // Memory is addressed through buffer resources over one domain block (base
 // `*_dom`, `*_size` elements): value (line, pos) of a source field is element
 // a + line*ls + pos*ps of the block (+ element b of the second array for
