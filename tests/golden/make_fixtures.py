@@ -10,9 +10,11 @@
      steps, 16 receivers within 120 cells of the source),
      sources1024_receivers.npz (the 256-source geometry: 1024 x 1024 cells as
      two 512 x 1024 domains, its first two seeded source positions with 8
-     seeded receivers each, the full 3000 steps, FP64)
+     seeded receivers each, the full 3000 steps, FP64),
+     lshape_frames.npz (the L-shaped scene's p0 of its three user domains after
+     the full 2000 steps, FP64 and FP32)
 
-    python tests/golden/make_fixtures.py [--kat] [--series] [--square] [--sources]
+    python tests/golden/make_fixtures.py [--kat] [--series] [--square] [--sources] [--frames]
 """
 import json
 import os
@@ -83,6 +85,21 @@ def sources(n_sources=2, steps=3000):
     np.savez(os.path.join(HERE, "sources1024_receivers.npz"), **out)
 
 
+def frames(steps=2000):
+    from oracle import binding as ob
+    from openpstd_amd import workloads as W
+    threads = int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1))
+    out = {}
+    for prec in (64, 32):
+        o = ob.OracleScene(W.lshape(0, steps), prec, threads=threads)
+        o.step(steps)
+        for i in range(o.n_domains):
+            if not o.domain_info(i)["is_pml"]:
+                out[f"fp{prec}_d{i}"] = o.field(i, "p0").astype(np.float64 if prec == 64 else np.float32)
+        print("frames", prec, "done", flush=True)
+    np.savez(os.path.join(HERE, "lshape_frames.npz"), **out)
+
+
 if __name__ == "__main__":
     args = sys.argv[1:] or ["--kat", "--series"]
     if "--kat" in args:
@@ -93,3 +110,5 @@ if __name__ == "__main__":
         series(("square1024",))
     if "--sources" in args:
         sources()
+    if "--frames" in args:
+        frames()

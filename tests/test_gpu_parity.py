@@ -111,6 +111,30 @@ def test_lshape_full_golden():
     assert np.isfinite(got).all()
 
 
+def test_lshape_final_frames_vs_golden():
+    """Pressure frames (p0) of the three user domains after the full 2000 steps
+    of the bench workload against the committed FP64 oracle frames, per domain
+    (rel-L2 and max-abs); the CPU FP32 oracle's error beside them."""
+    path = os.path.join(HERE, "golden", "lshape_frames.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated (tests/golden/make_fixtures.py --frames)")
+    g = np.load(path)
+    conf = W.lshape(0, 2000)
+    k, dev = device(conf)
+    dev.advance(2000)
+    doms = sorted(int(key[len("fp64_d"):]) for key in g.files if key.startswith("fp64_d"))
+    assert len(doms) == 3
+    for i in doms:
+        want = g[f"fp64_d{i}"]
+        got = dev.field(i, "p0")
+        assert got.shape == want.shape
+        e, m = rel(got, want), np.abs(got - want).max()
+        print(f"L-shape 2000 steps, domain {i} frame: rel-L2 vs FP64 {e:.2e}, max-abs {m:.2e} "
+              f"(CPU FP32 oracle {rel(g[f'fp32_d{i}'], want):.2e}); peak |p0| {np.abs(want).max():.2e}")
+        assert np.abs(want).max() > 1e-3
+        assert e <= 1e-4 and m <= 1e-6, (i, e, m)
+
+
 def test_room_full_golden():
     g = np.load(os.path.join(HERE, "golden", "room_receivers.npz"))
     conf = W.room(500)
