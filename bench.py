@@ -51,18 +51,19 @@ def workload(name, steps):
     from openpstd_amd import workloads as W
     if name == "lshape":
         return [W.lshape(0, steps)], ("L-shaped 3-domain scene, 3 x 256x128 cells + 50-cell PML, edge absorption "
-                                      "from {0,0.5,1} (seed 0), 4 receivers, FP32 FFT path")
+                                      "from {0,0.5,1} (seed 0), 4 receivers")
     if name == "room":
-        return [W.room(steps)], "single 100x50-cell domain + 50-cell PML, centre source, 1 receiver, FP32 FFT path"
+        return [W.room(steps)], "single 100x50-cell domain + 50-cell PML, centre source, 1 receiver"
     if name == "square4096":
-        return [W.square(0, steps)], "single 4096x4096-cell domain + 50-cell PML, 16 receivers, FP32 FFT path"
+        return [W.square(0, steps, receiver_radius=400)], ("single 4096x4096-cell domain + 50-cell PML, centre "
+                                                           "source, 16 receivers within 400 cells of it")
     if name == "sources256":
         return W.source_batch(0, steps), ("one 1024x1024-cell geometry (two 512x1024 domains + 50-cell PML) with 256 "
                                           "seeded source positions advanced together, 8 receivers each")
     if name == "batch64":
         return W.scene_batch(0, steps), ("64 independent scenes (seed 0) of 1-4 rectangular domains, sides 64-1024 "
                                          "cells, edge absorption U[0,1], 1 source + 1 receiver each, advanced "
-                                         "together (one launch per RK stage), FP32 FFT path")
+                                         "together (one launch per RK stage)")
     raise SystemExit(f"unknown workload {name}")
 
 

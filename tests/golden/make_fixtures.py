@@ -12,9 +12,11 @@
      two 512 x 1024 domains, its first two seeded source positions with 8
      seeded receivers each, the full 3000 steps, FP64),
      lshape_frames.npz (the L-shaped scene's p0 of its three user domains after
-     the full 2000 steps, FP64 and FP32)
+     the full 2000 steps, FP64 and FP32),
+     square4096_receivers.npz (the 4096 x 4096-cell domain of the bench, 16
+     seeded receivers within 400 cells of the centre source, 1000 steps, FP64)
 
-    python tests/golden/make_fixtures.py [--kat] [--series] [--square] [--sources] [--frames]
+    python tests/golden/make_fixtures.py [--kat] [--series] [--square] [--sources] [--frames] [--square4096]
 """
 import json
 import os
@@ -85,6 +87,15 @@ def sources(n_sources=2, steps=3000):
     np.savez(os.path.join(HERE, "sources1024_receivers.npz"), **out)
 
 
+def square4096(steps=1000):
+    from oracle import binding as ob
+    from openpstd_amd import workloads as W
+    threads = int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1))
+    conf = W.square(0, steps, n=4096, n_receivers=16, receiver_radius=400)
+    np.savez(os.path.join(HERE, "square4096_receivers.npz"), fp64=ob.OracleScene(conf, 64, threads=threads).step(steps))
+    print("square4096 done", flush=True)
+
+
 def frames(steps=2000):
     from oracle import binding as ob
     from openpstd_amd import workloads as W
@@ -112,3 +123,5 @@ if __name__ == "__main__":
         sources()
     if "--frames" in args:
         frames()
+    if "--square4096" in args:
+        square4096()
