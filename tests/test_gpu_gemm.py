@@ -159,3 +159,24 @@ def test_sources1024_per_source_vs_golden():
     for (form, i), (e, m) in errs.items():
         assert e <= 1e-4, (form, i, e)
         assert m <= 5e-6, (form, i, m)
+
+
+def test_square4096_forms_vs_golden():
+    """The 4096 x 4096-cell bench domain (F = 8192 lines), 1000 steps, 16
+    receivers within 400 cells of the centre source, against the committed FP64
+    oracle series; both forms within the 1e-4 bar."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "square4096_receivers.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated (tests/golden/make_fixtures.py --square4096)")
+    want = np.load(path)["fp64"]
+    steps = 1000
+    conf = W.square(0, steps, receiver_radius=400)
+    assert np.abs(want).max() > 1e-3
+    for form in ("gemm", "fft"):
+        _, dev = device(conf, form)
+        if form == "gemm":
+            assert dev.gemm_work()[0] > 0
+        got = dev.advance(steps)
+        e, m = rel(got, want), np.abs(got - want).max()
+        print(f"4096^2, {steps} steps, {form} form: receiver rel-L2 vs FP64 {e:.2e}, max-abs {m:.2e}")
+        assert e <= 1e-4 and m <= 5e-6, (form, e, m)
