@@ -202,6 +202,18 @@ __device__ __forceinline__ void fft_pass_twiddles(float2 (&w)[kTwPass], int t, c
     }
 }
 
+// Exchange point after pass PASS.  Rows of F <= 512 (T = F/8 <= 64 threads
+// per pair) keep every line pair inside one wavefront, and a pair's LDS slots
+// are its own; the LDS executes one wavefront's accesses in issue order, so a
+// compiler fence is enough there.  The first exchange of the forward transform
+// stays a workgroup barrier: it also separates every wavefront's reads of the
+// staged raw lines (which other pairs' slots overlap) from the writes of pass 1.
+template <int F, int LAYOUT, bool INV, int PASS>
+__device__ __forceinline__ void pass_sync() {
+    if constexpr (LAYOUT == 0 && F / 8 <= 64 && (INV || PASS > 0)) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    else __syncthreads();
+}
+
 // Transform with the twiddles of each pass loaded one pass ahead, into
 // registers: the loads of pass p+1 are in flight while pass p computes and
 // exchanges.  wcur: this pass's twiddles (unused by pass 0).
@@ -214,7 +226,7 @@ __device__ __forceinline__ void fft_run_ahead(float2 (&v)[8], int p, int t, floa
     float2 *out = (PASS & 1) ? buf1 : buf0;
     fft_pass<F, P, LAYOUT, PASS, INV, false, true>(v, p, t, out, wcur);
     if constexpr (PASS + 1 < NP) {
-        __syncthreads();
+        pass_sync<F, LAYOUT, INV, PASS>();
         lds_load<F, P, LAYOUT>(v, p, t, out);
         fft_run_ahead<F, P, LAYOUT, INV, PASS + 1>(v, p, t, buf0, buf1, tw, wnext);
     }
@@ -228,7 +240,7 @@ __device__ __forceinline__ void fft_run(float2 (&v)[8], int p, int t, float2 *bu
     float2 *out = (PASS & 1) ? buf1 : buf0;
     fft_pass<F, P, LAYOUT, PASS, INV, TWG>(v, p, t, out, w);
     if constexpr (PASS + 1 < NP) {
-        __syncthreads();
+        pass_sync<F, LAYOUT, INV, PASS>();
         lds_load<F, P, LAYOUT>(v, p, t, out);
         fft_run<F, P, LAYOUT, INV, TWG, PASS + 1>(v, p, t, buf0, buf1, w);
     }

@@ -396,7 +396,8 @@ __device__ __forceinline__ void derivative_fft(float2 (&v)[8], float2 (&h)[8], c
     }
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], h[m]);
-    __syncthreads();
+    // the inverse's first pass overwrites what the forward's last exchange read
+    pass_sync<F, LAYOUT, true, 0>();
     if constexpr (LEAN) {
         const float2 w0[kTwPass] = {};
         fft_run_ahead<F, P, LAYOUT, true>(v, p, t, buf0, buf1, tw, w0);
