@@ -204,13 +204,12 @@ __device__ __forceinline__ void fft_pass_twiddles(float2 (&w)[kTwPass], int t, c
 
 // Exchange point after pass PASS.  Rows of F <= 512 (T = F/8 <= 64 threads
 // per pair) keep every line pair inside one wavefront, and a pair's LDS slots
-// are its own; the LDS executes one wavefront's accesses in issue order, so a
-// compiler fence is enough there.  The first exchange of the forward transform
-// stays a workgroup barrier: it also separates every wavefront's reads of the
-// staged raw lines (which other pairs' slots overlap) from the writes of pass 1.
+// (in both buffers, and its staged raw lines) are its own; the LDS executes
+// one wavefront's accesses in issue order, so a compiler fence is enough
+// there.  Everything else synchronises the workgroup.
 template <int F, int LAYOUT, bool INV, int PASS>
 __device__ __forceinline__ void pass_sync() {
-    if constexpr (LAYOUT == 0 && F / 8 <= 64 && (INV || PASS > 0)) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if constexpr (LAYOUT == 0 && F / 8 <= 64) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     else __syncthreads();
 }
 

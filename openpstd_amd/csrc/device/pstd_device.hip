@@ -152,8 +152,11 @@ struct TileGeo {
     __device__ int pair(int tid) const { return LAYOUT == 0 ? tid >> logT : tid & (P - 1); }
     __device__ int lane_t(int tid) const { return LAYOUT == 0 ? tid & (T - 1) : tid >> logP; }
     // raw staging of the 2P lines in LDS (float): line-major for rows,
-    // position-major for columns, so that the wavefront's lanes stay contiguous
-    __device__ int line_base(int ll) const { return LAYOUT == 0 ? ll << (logT + 3) : ll; }
+    // position-major for columns, so that the wavefront's lanes stay contiguous.
+    // Rows use a pitch of 9F/8 floats: the two lines of pair p then fill exactly
+    // the slots pair p's transform uses in the second buffer (its own, so the
+    // transforms of rows need no workgroup barrier after the staging)
+    __device__ int line_base(int ll) const { return LAYOUT == 0 ? (ll << (logT + 3)) + (ll << logT) : ll; }
     __device__ int pos_step() const { return LAYOUT == 0 ? 1 : 2 * P; }
     __device__ int stage_at(int ll, int pos) const { return line_base(ll) + pos * pos_step(); }
 };
@@ -526,9 +529,10 @@ __device__ __forceinline__ void tile_layout(const StageArgs &a, const TileDesc &
     }
     UpdateRegs u;
     if constexpr (FL == kEarly) prefetch_updates<LAYOUT>(a, td, g, p, t, u);
-    // buf1 holds the raw lines (2P x F floats) and, after them, the window
+    // buf1 holds the raw lines (2P lines, at most 18 NT floats); the window
+    // has slots of its own (no pass of any pair writes them)
+    __shared__ float win[2 * 32 + 1];
     float *stage = reinterpret_cast<float *>(buf1);
-    float *win = stage + 16 * NT;
     float2 v[8];
     gather_finish<LAYOUT>(a, td, g, p, t, R, stage, win, v);
 #define PSTD_FFT_CASE(FV)                                                                                \
