@@ -14,9 +14,12 @@
      lshape_frames.npz (the L-shaped scene's p0 of its three user domains after
      the full 2000 steps, FP64 and FP32),
      square4096_receivers.npz (the 4096 x 4096-cell domain of the bench, 16
-     seeded receivers within 400 cells of the centre source, 1000 steps, FP64)
+     seeded receivers within 400 cells of the centre source, 1000 steps, FP64),
+     batch64_receivers.npz (four scenes of the 64-scene batch, seed 0: 19 with
+     one domain, 56 with two, 40 with three, 50 with four; 2000 steps, FP64)
 
     python tests/golden/make_fixtures.py [--kat] [--series] [--square] [--sources] [--frames] [--square4096]
+                                         [--batch64]
 """
 import json
 import os
@@ -96,6 +99,21 @@ def square4096(steps=1000):
     print("square4096 done", flush=True)
 
 
+BATCH64_SCENES = (19, 56, 40, 50)
+
+
+def batch64(steps=2000):
+    from oracle import binding as ob
+    from openpstd_amd import workloads as W
+    threads = int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1))
+    confs = W.scene_batch(0, steps)
+    out = {}
+    for i in BATCH64_SCENES:
+        out[f"fp64_scene{i}"] = ob.OracleScene(confs[i], 64, threads=threads).step(steps)
+        print("scene", i, "done", flush=True)
+    np.savez(os.path.join(HERE, "batch64_receivers.npz"), **out)
+
+
 def frames(steps=2000):
     from oracle import binding as ob
     from openpstd_amd import workloads as W
@@ -125,3 +143,5 @@ if __name__ == "__main__":
         frames()
     if "--square4096" in args:
         square4096()
+    if "--batch64" in args:
+        batch64()

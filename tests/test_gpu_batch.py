@@ -147,3 +147,29 @@ def test_tile_threads_bitwise(nx, ny):
     for d in range(k.domain_count()):
         for f in ("p0", "vx0", "vy0"):
             assert np.array_equal(a.field(d, f), b.field(d, f)), (d, f)
+
+
+def test_batch64_scenes_vs_golden():
+    """The 64-scene batch at full length (2000 steps, FP32 FFT form, one launch
+    per RK stage): four of its scenes (one to four domains) against committed
+    FP64 oracle series; the same four as a GEMM-form batch."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "batch64_receivers.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated (tests/golden/make_fixtures.py --batch64)")
+    g = np.load(path)
+    steps = 2000
+    confs = W.scene_batch(0, steps)
+    picked = sorted(int(k[len("fp64_scene"):]) for k in g.files)
+    assert len(picked) == 4
+    full = P.BatchSolver(confs)
+    got_full = full.advance(steps)
+    sub = P.BatchSolver([confs[i] for i in picked], form="gemm")
+    got_sub = sub.advance(steps)
+    for j, i in enumerate(picked):
+        want = g[f"fp64_scene{i}"]
+        assert np.abs(want).max() > 1e-3
+        for form, got in (("fft", full.scene_samples(got_full, i)), ("gemm", sub.scene_samples(got_sub, j))):
+            e, m = rel(got, want), np.abs(got - want).max()
+            print(f"batch scene {i} ({len(confs[i].Domains)} domains), {form} form, {steps} steps: "
+                  f"rel-L2 vs FP64 {e:.2e}, max-abs {m:.2e}")
+            assert e <= 1e-4 and m <= 5e-6, (i, form, e, m)
