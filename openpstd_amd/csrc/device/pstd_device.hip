@@ -1748,7 +1748,9 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     ctx->steps_done = 0;
     ctx->p0_explicit = true;
     ctx->planned = true;
-    return autotune(ctx);
+    const int rc = autotune(ctx);
+    if (rc) ctx->planned = false;  // a failed trial run leaves no usable launch state
+    return rc;
 }
 
 int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
@@ -1984,7 +1986,10 @@ int pstd_ctx_set_tile_threads(pstd_ctx *ctx, int threads) {
     ctx->drop_graph();
     ctx->small_nt = threads;
     const int rc = upload_tiles(ctx);
-    if (rc) return rc;
+    if (rc) {
+        ctx->planned = false;  // the tile lists are gone with the failed upload
+        return rc;
+    }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return PSTD_OK;
 }
