@@ -157,7 +157,7 @@ def main():
                     help="workgroup-to-tile order (default: the autotuner's choice)")
     ap.add_argument("--flavour", default="auto", choices=["auto", "early", "late", "mid"],
                     help="stage-kernel flavour (default: the autotuner's choice)")
-    ap.add_argument("--tile-threads", default="auto", choices=["auto", "256", "512"],
+    ap.add_argument("--tile-threads", default="auto", choices=["auto", "128", "256", "512"],
                     help="block size for lines of FFT length <= 2048 (default: the autotuner's choice)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-compare", type=int, default=0, metavar="N",

@@ -216,8 +216,10 @@ int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps);
  *    throughput-bound scenes), 2 loads filter and twiddles up front and t_n
  *    where used;
  *  - tile threads: block size of the kernel for lines of FFT length <= 2048,
- *    256 or 512 (512 puts twice the lines in a tile: for columns, twice the
- *    contiguous bytes of each row per wave access).
+ *    128, 256 or 512 (512 puts twice the lines of 256 in a tile: for columns,
+ *    twice the contiguous bytes of each row per wave access; 128 spreads a
+ *    small scene over more CUs and needs every such line to have F <= 1024,
+ *    else PSTD_ERR_INVALID).
  * pstd_ctx_get_tuning reports the choice and its timed GPU ms per stage,
  * pstd_ctx_get_tile_threads the block size. */
 int pstd_ctx_set_tile_order(pstd_ctx *ctx, int xcd_contiguous);

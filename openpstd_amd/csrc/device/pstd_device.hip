@@ -608,9 +608,10 @@ __device__ __forceinline__ int tile_index(int b, int n, int remap) {
     return x * q + min(x, r) + l;
 }
 
-// F <= 2048 class, NT = 256 or 512 threads: two LDS buffers of P * F = 8 * NT
-// complex values each (padded).  Wider blocks hold more lines per tile (for
-// columns, more of each row per wave access) at the same waves per CU.
+// F <= 2048 class, NT = 128, 256 or 512 threads (128: F <= 1024 only): two LDS
+// buffers of P * F = 8 * NT complex values each (padded).  Wider blocks hold
+// more lines per tile (for columns, more of each row per wave access) at the
+// same waves per CU; narrower ones spread a small scene over more CUs.
 template <int NT, int FL>
 __global__ void __launch_bounds__(NT) stage_small_kernel(StageArgs a) {
     constexpr int kBuf = lds_padded(8 * NT);
@@ -625,7 +626,7 @@ __global__ void __launch_bounds__(NT) stage_small_kernel(StageArgs a) {
         passthrough_tile<NT>(a, td);
         return;
     }
-    derivative_tile<NT, 128, 2048, FL>(a, td, lds, lds + kBuf);
+    derivative_tile<NT, 128, (NT >= 256 ? 2048 : 1024), FL>(a, td, lds, lds + kBuf);
 }
 
 template <int F, int FL>
@@ -978,6 +979,7 @@ struct pstd_ctx {
     // block size of the F <= 2048 class (small_nt, autotuned: 256 or 512)
     std::vector<TileDesc> fft_segs, tiles_extra, tiles_all;
     int small_nt = 256;
+    int small_fmax = 0;  // largest F of the F <= 2048 class (128 threads need F <= 1024)
     std::vector<GemmDesc> tiles_gemm, tiles_split;
     std::vector<SplitPiece> pieces;
     long long u_halfs = 0, u_exps = 0;  // split-input scratch of the GEMM form
@@ -1119,6 +1121,10 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
             if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<512, kLate>), dim3(ns), dim3(512), 0, ctx->stream, a);
             else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<512, kMid>), dim3(ns), dim3(512), 0, ctx->stream, a);
             else hipLaunchKernelGGL((stage_small_kernel<512, kEarly>), dim3(ns), dim3(512), 0, ctx->stream, a);
+        } else if (ctx->small_nt == 128) {
+            if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<128, kLate>), dim3(ns), dim3(128), 0, ctx->stream, a);
+            else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<128, kMid>), dim3(ns), dim3(128), 0, ctx->stream, a);
+            else hipLaunchKernelGGL((stage_small_kernel<128, kEarly>), dim3(ns), dim3(128), 0, ctx->stream, a);
         } else {
             if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<256, kLate>), dim3(ns), dim3(256), 0, ctx->stream, a);
             else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<256, kMid>), dim3(ns), dim3(256), 0, ctx->stream, a);
@@ -1316,12 +1322,16 @@ static int autotune(pstd_ctx *ctx) {
         return rc;
     };
     float best = 0;
-    int bi = 0, bnt = 256, rc = PSTD_OK;
-    bool has_small = false;
-    for (const TileDesc &g : ctx->fft_segs) has_small = has_small || g.F <= 2048;
-    for (int nt = 256; nt <= 512 && rc == PSTD_OK; nt *= 2) {
+    int bi = 0, bnt = ctx->small_nt, rc = PSTD_OK;
+    bool first = true;
+    // block sizes of the F <= 2048 class (it alone depends on them)
+    std::vector<int> nts = {ctx->small_nt};
+    if (ctx->small_fmax > 0)
+        for (int nt : {128, 256, 512})
+            if (nt != ctx->small_nt && (nt >= 256 || ctx->small_fmax <= 1024)) nts.push_back(nt);
+    for (int nt : nts) {
+        if (rc != PSTD_OK) break;
         if (nt != ctx->small_nt) {
-            if (!has_small) break;  // the block size only shapes the F <= 2048 class
             ctx->small_nt = nt;
             rc = upload_tiles(ctx);
         }
@@ -1330,10 +1340,11 @@ static int autotune(pstd_ctx *ctx) {
             ctx->xcd_remap = cand[c][1];
             float ms = 0;
             rc = time_steps(ms);
-            if (rc == PSTD_OK && ((nt == 256 && c == 0) || ms < best)) {
+            if (rc == PSTD_OK && (first || ms < best)) {
                 best = ms;
                 bi = c;
                 bnt = nt;
+                first = false;
             }
         }
     }
@@ -1436,6 +1447,8 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     ctx->release();
     ctx->doms.clear();
     ctx->tiles_small.clear();
+    ctx->small_fmax = 0;
+    ctx->small_nt = 256;
     ctx->fft_segs.clear();
     ctx->tiles_extra.clear();
     ctx->tiles_4096.clear();
@@ -1638,6 +1651,7 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         }
         g.line0 = s.line_begin;
         g.count = s.line_end - s.line_begin;
+        if (g.F <= 2048) ctx->small_fmax = std::max(ctx->small_fmax, g.F);
         ctx->fft_segs.push_back(g);
     }
     // passthrough tiles for fields no segment updates (rigid domains, directions
@@ -1980,7 +1994,11 @@ int pstd_ctx_get_tuning(const pstd_ctx *ctx, int32_t *lean, int32_t *xcd_contigu
 }
 
 int pstd_ctx_set_tile_threads(pstd_ctx *ctx, int threads) {
-    if (!ctx || !ctx->planned || (threads != 256 && threads != 512)) return PSTD_ERR_INVALID;
+    if (!ctx || !ctx->planned || (threads != 128 && threads != 256 && threads != 512)) return PSTD_ERR_INVALID;
+    if (threads == 128 && ctx->small_fmax > 1024) {
+        ctx->err = "128-thread tiles need every line of the F <= 2048 class to have F <= 1024";
+        return PSTD_ERR_INVALID;
+    }
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->drop_graph();

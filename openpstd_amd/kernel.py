@@ -240,7 +240,7 @@ class _Context:
         _check(lib().pstd_ctx_set_kernel_flavour(self.ctx, int(f)), self._err)
 
     def set_tile_threads(self, threads):
-        """Block size (256 or 512) of the stage kernel for lines of FFT length <= 2048."""
+        """Block size (128, 256 or 512; 128 needs F <= 1024) of the stage kernel for lines of FFT length <= 2048."""
         _check(lib().pstd_ctx_set_tile_threads(self.ctx, int(threads)), self._err)
 
     def tuning(self):

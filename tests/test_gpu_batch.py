@@ -90,7 +90,7 @@ def test_graph_replay_is_bitwise_eager(graph_steps):
 
 
 @pytest.mark.parametrize("shape", [(0, 1, 256), (1, 0, 256), (1, 1, 256), (2, 0, 256), (2, 1, 256), (0, 0, 512),
-                                   (1, 1, 512), (2, 1, 512)])
+                                   (1, 1, 512), (2, 1, 512), (1, 1, 128), (0, 0, 128), (2, 1, 128)])
 def test_launch_shape_is_bitwise_neutral(shape):
     """Every (kernel flavour, tile order, tile threads) the autotuner may pick gives the same bits."""
     conf = W.lshape(2, 25)
@@ -127,9 +127,10 @@ def test_long_lines_lean_flavour_bitwise():
 
 @pytest.mark.parametrize("nx, ny", [(301, 1000), (1000, 301), (77, 129)])
 def test_tile_threads_bitwise(nx, ny):
-    """Block sizes 256 and 512 of the F <= 2048 class give the same bits: the
-    line pairs are the same, only the number of pairs per tile differs (odd
-    line counts leave a lone last line in both)."""
+    """Block sizes 128 (where every line has F <= 1024), 256 and 512 of the
+    F <= 2048 class give the same bits: the line pairs are the same, only the
+    number of pairs per tile differs (odd line counts leave a lone last line
+    in all)."""
     conf = P.PSTDConfiguration()
     W.base_settings(conf, 4)
     conf.Domains = [W.domain(0, 0, nx, ny, 1.0)]
@@ -140,13 +141,23 @@ def test_tile_threads_bitwise(nx, ny):
     b.set_tile_threads(512)
     assert b.tuning()["tile_threads"] == 512
     with pytest.raises(P.PSTDError):
-        b.set_tile_threads(384)  # only 256 and 512 exist
+        b.set_tile_threads(384)  # only 128, 256 and 512 exist
     assert b.tuning()["tile_threads"] == 512
+    small = max(nx, ny) + 1 + 64 <= 1024  # every line of FFT length <= 1024
+    _, c = single(conf)
+    if small:
+        c.set_tile_threads(128)
+    else:
+        with pytest.raises(P.PSTDError):
+            c.set_tile_threads(128)
+        c.set_tile_threads(256)
     a.advance(4)
     b.advance(4)
+    c.advance(4)
     for d in range(k.domain_count()):
         for f in ("p0", "vx0", "vy0"):
             assert np.array_equal(a.field(d, f), b.field(d, f)), (d, f)
+            assert np.array_equal(a.field(d, f), c.field(d, f)), (d, f)
 
 
 def test_batch64_scenes_vs_golden():
