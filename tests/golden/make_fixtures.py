@@ -14,7 +14,7 @@
      lshape_frames.npz (the L-shaped scene's p0 of its three user domains after
      the full 2000 steps, FP64 and FP32),
      square4096_receivers.npz (the 4096 x 4096-cell domain of the bench, 16
-     seeded receivers within 400 cells of the centre source, 1000 steps, FP64),
+     seeded receivers within 200 cells of the centre source, 500 steps, FP64),
      batch64_receivers.npz (four scenes of the 64-scene batch, seed 0: 19 with
      one domain, 56 with two, 40 with three, 50 with four; 2000 steps, FP64)
 
@@ -90,11 +90,11 @@ def sources(n_sources=2, steps=3000):
     np.savez(os.path.join(HERE, "sources1024_receivers.npz"), **out)
 
 
-def square4096(steps=1000):
+def square4096(steps=500):
     from oracle import binding as ob
     from openpstd_amd import workloads as W
     threads = int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1))
-    conf = W.square(0, steps, n=4096, n_receivers=16, receiver_radius=400)
+    conf = W.square(0, steps, n=4096, n_receivers=16, receiver_radius=200)
     np.savez(os.path.join(HERE, "square4096_receivers.npz"), fp64=ob.OracleScene(conf, 64, threads=threads).step(steps))
     print("square4096 done", flush=True)
 

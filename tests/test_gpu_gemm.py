@@ -162,15 +162,15 @@ def test_sources1024_per_source_vs_golden():
 
 
 def test_square4096_forms_vs_golden():
-    """The 4096 x 4096-cell bench domain (F = 8192 lines), 1000 steps, 16
-    receivers within 400 cells of the centre source, against the committed FP64
+    """The 4096 x 4096-cell bench domain (F = 8192 lines), 500 steps, 16
+    receivers within 200 cells of the centre source, against the committed FP64
     oracle series; both forms within the 1e-4 bar."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "square4096_receivers.npz")
     if not os.path.exists(path):
         pytest.skip("fixture not generated (tests/golden/make_fixtures.py --square4096)")
     want = np.load(path)["fp64"]
-    steps = 1000
-    conf = W.square(0, steps, receiver_radius=400)
+    steps = 500
+    conf = W.square(0, steps, receiver_radius=200)
     assert np.abs(want).max() > 1e-3
     for form in ("gemm", "fft"):
         _, dev = device(conf, form)
