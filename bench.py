@@ -88,12 +88,11 @@ def plain_sequential(confs, n, steps):
             "form": "fft"}
 
 
-def cpu_baseline(confs, seconds=10.0, max_steps=400):
-    """FP32 oracle on the host cores: MultiThreadSolver-style OpenMP over the
-    calc tasks, timed over a bounded sample of the same workload (whole steps
-    of the first scenes of a batch)."""
+def _oracle_rate(confs, threads, seconds, max_steps):
+    """cell-RK-stage/s of the FP32 oracle with `threads` OpenMP threads over a
+    bounded sample: whole steps of one scene (or 2 steps of the first scenes
+    of a batch) until `seconds` of timed work."""
     from oracle import binding as ob
-    threads = min(16, os.cpu_count() or 1)
     cell_steps, elapsed, steps_run, scenes = 0, 0.0, 0, 0
     for conf in confs:
         o = ob.OracleScene(conf, 32, threads)
@@ -115,8 +114,19 @@ def cpu_baseline(confs, seconds=10.0, max_steps=400):
             break
     what = (f"{steps_run} time steps of the same scene after 1 warm-up step" if len(confs) == 1 else
             f"2 time steps (after 1 warm-up step) of each of the first {scenes} of {len(confs)} scenes")
-    return {"value": cell_steps * 6 / elapsed, "unit": UNIT, "cores": threads, "kind": "port",
-            "sample": f"{what} ({elapsed:.1f} s timed), FP32 oracle, {threads} OpenMP threads"}
+    return cell_steps * 6 / elapsed, f"{what} ({elapsed:.1f} s timed), FP32 oracle, {threads} OpenMP thread(s)"
+
+
+def cpu_baseline(confs, seconds=10.0, max_steps=400):
+    """FP32 oracle on the host cores: MultiThreadSolver-style OpenMP over the
+    calc tasks (the reported value), and single-threaded as
+    SingleThreadSolver, each timed over a bounded sample of the same
+    workload."""
+    threads = min(16, os.cpu_count() or 1)
+    value, sample = _oracle_rate(confs, threads, seconds, max_steps)
+    single, single_sample = _oracle_rate(confs, 1, seconds / 2, max_steps)
+    return {"value": value, "unit": UNIT, "cores": threads, "kind": "port", "sample": sample,
+            "single_thread": {"value": single, "cores": 1, "sample": single_sample}}
 
 
 def measured_traffic(name):
