@@ -204,17 +204,18 @@ int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int
  * on first use after the first time step and dropped on re-upload.  Results
  * are identical either way. */
 int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps);
-/* Launch shape of the stage kernel.  pstd_ctx_upload_plan(s) times the
- * candidates on the zeroed state and keeps the fastest (the autotuner); these
- * calls override or report its choice.  Results are identical for every
- * choice.
+/* Launch shape of the stage kernel (no counterpart in the reference, whose
+ * FFTW plans are made with FFTW_ESTIMATE on every call,
+ * kernel/core/kernel_functions.cpp:175-186).  pstd_ctx_upload_plan(s) times
+ * the candidates on the zeroed state and keeps the fastest (the autotuner);
+ * these calls override or report its choice.  Results are identical for
+ * every choice.
  *  - tile order: 1 gives every XCD a contiguous run of tiles (neighbouring
  *    lines share its L2), 0 the plain round-robin order;
  *  - kernel flavour: 0 loads every descriptor-dependent operand (filter,
- *    twiddles, t_n) before the transforms (short tiles, latency-bound scenes),
- *    1 loads them where they are used (fewer registers, more resident waves,
- *    throughput-bound scenes), 2 loads filter and twiddles up front and t_n
- *    where used;
+ *    twiddles, t_n) before the transforms (most registers), 1 loads them
+ *    where they are used (fewest registers, most resident waves), 2 loads
+ *    filter and twiddles up front and t_n where used;
  *  - tile threads: block size of the kernel for lines of FFT length <= 2048,
  *    128, 256 or 512 (512 puts twice the lines of 256 in a tile: for columns,
  *    twice the contiguous bytes of each row per wave access; 128 spreads a
