@@ -1268,10 +1268,6 @@ static int enqueue_steps(pstd_ctx *ctx, int chunk) {
     return PSTD_OK;
 }
 
-// Launch-shape autotuner.  Times one stage-kernel flavour x tile order per
-// candidate on the freshly uploaded (all-zero) state and keeps the fastest.
-// Zero fields stay exactly zero under the update, and the arenas are cleared
-// again afterwards, so tuning leaves no trace in the results.
 // Tile lists for the current block size of the F <= 2048 class: a segment's
 // lines in tiles of NL = 2 P lines (P = NT / (F/8) pairs; one pair per tile
 // above 2048), then the passthrough/receiver tiles; uploaded as
@@ -1306,16 +1302,21 @@ static int upload_tiles(pstd_ctx *ctx) {
     return PSTD_OK;
 }
 
+// Launch-shape autotuner.  Times every (block size, stage-kernel flavour, tile
+// order) candidate on the freshly uploaded (all-zero) state and keeps the
+// fastest.  Zero fields stay exactly zero under the update, and the arenas are
+// cleared again afterwards, so tuning leaves no trace in the results.
 static int autotune(pstd_ctx *ctx) {
     static const int cand[6][2] = {{kLate, 1}, {kMid, 1}, {kEarly, 1}, {kLate, 0}, {kMid, 0}, {kEarly, 0}};  // (flavour, xcd order)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    // one warm-up step, then GPU ms of three steps
+    // one warm-up step, then GPU ms of n steps
+    int n = 3;
     auto time_steps = [&](float &ms) -> int {
         int rc = enqueue_steps(ctx, 1);
         if (rc == PSTD_OK && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = PSTD_ERR_DEVICE;
-        if (rc == PSTD_OK) rc = enqueue_steps(ctx, 3);
+        if (rc == PSTD_OK) rc = enqueue_steps(ctx, n);
         if (rc == PSTD_OK && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = PSTD_ERR_DEVICE;
         if (rc == PSTD_OK && hipEventSynchronize(e1) != hipSuccess) rc = PSTD_ERR_DEVICE;
         if (rc == PSTD_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PSTD_ERR_DEVICE;
@@ -1324,6 +1325,14 @@ static int autotune(pstd_ctx *ctx) {
     float best = 0;
     int bi = 0, bnt = ctx->small_nt, rc = PSTD_OK;
     bool first = true;
+    {
+        // small (latency-bound) scenes get more timed steps per candidate
+        // (about 2 ms of GPU time each), so that run-to-run noise does not
+        // pick the shape; large ones keep 3
+        float ms = 0;
+        rc = time_steps(ms);
+        if (rc == PSTD_OK && ms > 0.f) n = std::min(50, std::max(3, (int)std::ceil(2.0f / (ms / 3.0f))));
+    }
     // block sizes of the F <= 2048 class (it alone depends on them)
     std::vector<int> nts = {ctx->small_nt};
     if (ctx->small_fmax > 0)
@@ -1369,7 +1378,7 @@ static int autotune(pstd_ctx *ctx) {
         if (ctx->err.empty()) ctx->err = "autotune: HIP event failure";
         return rc;
     }
-    ctx->tuned_stage_ms = best / 18.0;
+    ctx->tuned_stage_ms = best / (6.0 * n);
     HIP_TRY(hipMemsetAsync(ctx->arena, 0, 4 * (size_t)ctx->arena_floats * sizeof(float), ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->steps_done = 0;
