@@ -394,8 +394,17 @@ void SceneModel::add_receiver(float x, float y, int mode) {
     r.cell_x = (int)x;
     r.cell_y = (int)y;
     int relx = r.cell_x - d.tlx, rely = r.cell_y - d.tly;
-    int row = mode == PSTD_RECEIVER_REFERENCE ? relx : rely;
-    int col = mode == PSTD_RECEIVER_REFERENCE ? rely : relx;
+    int row = rely, col = relx;
+    if (mode == PSTD_RECEIVER_REFERENCE) {
+        // p0(rel.x, rel.y) of openPSTD's column-major ny x nx array: element
+        // rel.x + rel.y * ny of its storage, without the bounds check of an
+        // NDEBUG build (for rel.x < ny the transposed cell)
+        const long long lin = (long long)relx + (long long)rely * d.ny;
+        if (relx < 0 || rely < 0 || lin >= (long long)d.nx * d.ny)
+            throw PSTDSolverError("receiver: openPSTD's p0(rel.x, rel.y) read lies outside its domain's pressure array");
+        row = (int)(lin % d.ny);
+        col = (int)(lin / d.ny);
+    }
     if (row < 0 || row >= d.ny || col < 0 || col >= d.nx)
         throw PSTDSolverError("receiver cell outside its domain's pressure array");
     r.index = row * d.nx + col;
@@ -416,6 +425,12 @@ void SceneModel::build_segments() {
                 std::vector<char> left(own.b - own.a, 1);  // lines not yet assigned
                 const std::vector<int> &n1 = X ? d.nb[kLeft] : d.nb[kBottom];
                 const std::vector<int> &n2 = X ? d.nb[kRight] : d.nb[kTop];
+                // openPSTD's Domain::calc sets d1 = shared_from_this() in the
+                // first segment without a side-1 neighbour and never resets it
+                // inside the j loop (kernel/core/Domain.cpp:148-150,210-233), so
+                // a second such segment would read the domain's own field as its
+                // side 1 with rho_self.  Scenes reaching that are rejected.
+                int no_side1_segments = 0;
                 for (size_t i = 0; i <= n1.size(); ++i) {
                     int d1 = i < n1.size() ? n1[i] : -1;
                     for (size_t j = 0; j <= n2.size(); ++j) {
@@ -440,6 +455,11 @@ void SceneModel::build_segments() {
                                                   std::to_string(d.id) + ", lines " + std::to_string(lo) + ".." +
                                                   std::to_string(hi) + ", " + std::to_string(count) + " free)");
                         for (int v = lo; v <= hi; ++v) left[v - own.a] = 0;
+                        if (d1 < 0 && ++no_side1_segments > 1)
+                            throw PSTDSolverError(
+                                "two line segments without a side-1 neighbour in one Domain::calc pass (domain " +
+                                std::to_string(d.id) + ", " + (X ? "X" : "Y") +
+                                "): openPSTD reuses the domain itself as side 1 there; not supported");
                         int primary = X ? d.nx : d.ny;
                         int wlen = window_;
                         while (wlen > primary) wlen /= 2;

@@ -29,6 +29,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
         L = C.CDLL(LIB_PATH)
+        L.oracle_set_mutation.argtypes = [C.c_int]
         L.oracle_next_2_power_int.argtypes = [C.c_int]
         L.oracle_next_2_power_float.argtypes = [C.c_float]
         L.oracle_rho_array.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]
@@ -64,6 +65,12 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def set_mutation(code):
+    """Plants a known mistake in the oracle (0 = none), for tests that show an
+    invariant detects it (see oracle/pstd_oracle.cpp)."""
+    lib().oracle_set_mutation(int(code))
 
 
 def next_2_power(n):

@@ -43,6 +43,14 @@ namespace orc {
 
 static thread_local std::string g_error;
 
+// Planted mistakes for the oracle's own invariant tests (tests/test_oracle_physics.py):
+// 0 = the restatement as written; 1 = side-1 halo read one cell off;
+// 2 = reflection coefficients of get_rho_array negated; 3 = PML attenuation
+// profiles reversed; 4 = Y lines take the top neighbour as side 1 (the
+// geometric order) instead of the bottom one.  Set only through
+// oracle_set_mutation, by tests, to show that the invariants fail on them.
+static int g_mutation = 0;
+
 enum Direction { LEFT = 0, RIGHT = 1, TOP = 2, BOTTOM = 3 };
 // kernel/core/kernel_functions.h:92-93
 static const Direction kAllDirections[4] = {LEFT, RIGHT, TOP, BOTTOM};
@@ -109,6 +117,12 @@ static RhoArray<R> get_rho_array(R rho1, R rho_self, R rho2) {
     R tlw1 = (2 * zn1) / (zn1 + 1), tlw2 = (2 * inv_zn1) / (inv_zn1 + 1);
     R zn2 = rho2 / rho_self, inv_zn2 = rho_self / rho2;
     R rrw1 = (zn2 - 1) / (zn2 + 1), rrw2 = (inv_zn2 - 1) / (inv_zn2 + 1);
+    if (g_mutation == 2) {
+        rlw1 = -rlw1;
+        rlw2 = -rlw2;
+        rrw1 = -rrw1;
+        rrw2 = -rrw2;
+    }
     R trw1 = (2 * zn2) / (zn2 + 1), trw2 = (2 * inv_zn2) / (inv_zn2 + 1);
     RhoArray<R> r;
     R p[4][2] = {{rlw1, rlw2}, {rrw1, rrw2}, {tlw1, tlw2}, {trw1, trw2}};
@@ -305,12 +319,13 @@ static Mat<R> spatderp3(Mat<R> p1, Mat<R> p2, Mat<R> p3, const std::vector<std::
     std::vector<std::complex<R>> spec(F / 2 + 1), scratch;
     const R *wl = window.data();
     const R *wr = window.data() + wlen + 1;
+    const int m1 = g_mutation == 1 ? 1 : 0;
     for (int r = 0; r < batch; ++r) {
         std::fill(line.begin(), line.end(), R(0));
         if (ct == PRESSURE) {
             if (p1.cols < wlen || p3.cols < wlen) throw std::runtime_error("window bigger than sides");
             for (int j = 0; j < wlen; ++j) {
-                R d1 = p1(r, p1.cols - wlen + j) * rho.p[2][1] + p2(r, wlen - 1 - j) * rho.p[0][0];
+                R d1 = p1(r, p1.cols - wlen + j - m1) * rho.p[2][1] + p2(r, wlen - 1 - j) * rho.p[0][0];
                 line[j] = d1 * wl[j];
                 R d3 = p3(r, j) * rho.p[3][1] + p2(r, n2 - 1 - j) * rho.p[1][0];
                 line[wlen + n2 + j] = d3 * wr[j];
@@ -318,7 +333,7 @@ static Mat<R> spatderp3(Mat<R> p1, Mat<R> p2, Mat<R> p3, const std::vector<std::
         } else {
             if (p1.cols < wlen + 1 || p3.cols < wlen + 1) throw std::runtime_error("window bigger than sides");
             for (int j = 0; j < wlen; ++j) {
-                R d1 = p1(r, p1.cols - wlen - 1 + j) * rho.v[2][1] + p2(r, wlen - j) * rho.v[0][0];
+                R d1 = p1(r, p1.cols - wlen - 1 + j - m1) * rho.v[2][1] + p2(r, wlen - j) * rho.v[0][0];
                 line[j] = d1 * wl[j];
                 R d3 = p3(r, 1 + j) * rho.v[3][1] + p2(r, n2 - 2 - j) * rho.v[1][0];
                 line[wlen + n2 + j] = d3 * wr[j];
@@ -794,8 +809,12 @@ struct Scene {
         Point rel = r.grid - r.container->top_left;
         int ny = r.container->size.y, nx = r.container->size.x;
         if (receiver_mode == PSTD_RECEIVER_REFERENCE) {
-            *row = rel.x;
-            *col = rel.y;
+            // Eigen's p0(rel.x, rel.y) on the column-major ny x nx array reads
+            // storage element rel.x + rel.y * ny (unchecked in an NDEBUG build)
+            long lin = (long)rel.x + (long)rel.y * ny;
+            if (rel.x < 0 || rel.y < 0 || lin >= (long)nx * ny) return false;
+            *row = (int)(lin % ny);
+            *col = (int)(lin / ny);
         } else {
             *row = rel.y;
             *col = rel.x;
@@ -827,7 +846,7 @@ void Domain<R>::create_attenuation_array(CalcDir cd, bool ascending, Mat<R> &pp,
         R a = rho_air * ampmax * std::pow(r, (R)4);
         fv[i] = std::exp(-a * dt);
     }
-    if (!ascending) {
+    if (!ascending != (g_mutation == 3)) {
         std::reverse(fp.begin(), fp.end());
         std::reverse(fv.begin(), fv.end());
     }
@@ -878,9 +897,15 @@ template <class R>
 void Domain<R>::calc(CalcDir cd, CalcType ct) {
     Scene<R> &sc = *scene;
     std::vector<int> own = get_range(cd);
-    std::vector<Domain *> &d1s = cd == X ? left : bottom;
-    std::vector<Domain *> &d2s = cd == X ? right : top;
+    const bool geometric_y = g_mutation == 4;
+    std::vector<Domain *> &d1s = cd == X ? left : (geometric_y ? top : bottom);
+    std::vector<Domain *> &d2s = cd == X ? right : (geometric_y ? bottom : top);
     Mat<R> &source = ct == PRESSURE ? (cd == X ? L.Lpx : L.Lpy) : (cd == X ? L.Lvx : L.Lvy);
+    // Domain.cpp:148-150,210-233: the first segment without a side-1
+    // neighbour sets d1 = shared_from_this(), which stays set for the rest of
+    // the j loop; a second such segment would read the domain itself as side 1.
+    // Rejected (no scene of the supported set reaches it).
+    int no_side1_segments = 0;
     for (size_t i = 0; i != d1s.size() + 1; ++i) {
         Domain *d1 = i != d1s.size() ? d1s[i] : nullptr;
         for (size_t j = 0; j != d2s.size() + 1; ++j) {
@@ -897,6 +922,9 @@ void Domain<R>::calc(CalcDir cd, CalcType ct) {
                 inter = t;
             }
             if (inter.empty()) continue;
+            if (!d1 && ++no_side1_segments > 1)
+                throw std::runtime_error("two segments without a side-1 neighbour in one Domain::calc pass "
+                                         "(openPSTD reuses the domain itself as side 1)");
             {
                 std::vector<int> diff;
                 std::set_difference(own.begin(), own.end(), inter.begin(), inter.end(),
@@ -1123,6 +1151,7 @@ static std::string g_last_error;
 
 // Exported functions take C linkage from their declarations in pstd_oracle.h.
 
+void oracle_set_mutation(int code) { g_mutation = code; }
 int oracle_next_2_power_int(int n) { return next_2_power_int(n); }
 int oracle_next_2_power_float(float n) { return next_2_power_float(n); }
 

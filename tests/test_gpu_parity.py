@@ -251,6 +251,26 @@ def test_reference_receiver_mode():
     assert_series(got, want)
 
 
+def test_room_receiver_both_modes():
+    """The 100 x 50-cell room (500 steps, receiver at world (15, 7) m = cell
+    (75, 35)) in both receiver modes against the FP64 oracle.  REFERENCE mode
+    reads openPSTD's p0(75, 35) of the column-major 50 x 100 array: storage
+    element 75 + 35 * 50 = 1825, the cell y = 25, x = 36."""
+    out = {}
+    for mode in (T.PSTD_RECEIVER_PHYSICAL, T.PSTD_RECEIVER_REFERENCE):
+        conf = W.room(500)
+        conf.receiver_mode = mode
+        k, dev = device(conf)
+        got = dev.advance(500)
+        want = ob.OracleScene(conf, 64, THREADS).step(500)
+        assert np.abs(want).max() > 1e-2
+        assert_series(got, want)
+        out[mode] = (got, want)
+        print(f"room, receiver mode {mode}: rel-L2 vs FP64 "
+              f"{np.linalg.norm(got - want) / np.linalg.norm(want):.2e}, peak {np.abs(want).max():.3f}")
+    assert not np.allclose(out[T.PSTD_RECEIVER_PHYSICAL][1], out[T.PSTD_RECEIVER_REFERENCE][1])
+
+
 def test_set_field_midrun_p0():
     """Host writes between steps keep openPSTD semantics (p0 is its own array)."""
     conf = W.room(10)

@@ -115,12 +115,30 @@ def test_receiver_modes():
         d, idx = o.receiver_cell(0)
         assert d == 0
         assert idx == (30 * 256 + 40 if mode == T.PSTD_RECEIVER_PHYSICAL else 40 * 256 + 30)
-    # openPSTD's transposed read of a cell outside the array is an error
+    # rel.x >= ny: openPSTD's p0(rel.x, rel.y) reads storage element
+    # rel.x + rel.y * ny of the column-major 128 x 256 array (unchecked in an
+    # NDEBUG build): 200 + 30 * 128 = 4040 -> row 4040 % 128 = 72, column 31
+    conf.receiver_mode = T.PSTD_RECEIVER_REFERENCE
     conf.Receivers = [(W._world_pos(200.25), W._world_pos(30.25), 0.0)]
+    k = P.PSTDKernel()
+    k.initialize_kernel(conf)
+    assert ob.OracleScene(conf, 32).receiver_cell(0) == (0, 72 * 256 + 31)
+    # the room's receiver (75, 35) in its 100 x 50 domain: 75 + 35 * 50 = 1825 -> row 25, column 36
+    room = W.room(2)
+    room.receiver_mode = T.PSTD_RECEIVER_REFERENCE
+    o = ob.OracleScene(room, 32)
+    (ui,) = [i for i in range(o.n_domains) if not o.domain_info(i)["is_pml"]]
+    assert o.receiver_cell(0) == (ui, 25 * 100 + 36)
+    # past the end of the storage (a tall 60 x 120 domain: 50 + 100 * 120 >= 7200) is an error
+    tall = P.PSTDConfiguration()
+    W.base_settings(tall, 2)
+    tall.Domains = [W.domain(0, 0, 60, 120, 1.0)]
+    tall.receiver_mode = T.PSTD_RECEIVER_REFERENCE
+    tall.Receivers = [(W._world_pos(50.25), W._world_pos(100.25), 0.0)]
     with pytest.raises(P.PSTDError, match="receiver"):
-        P.PSTDKernel().initialize_kernel(conf)
+        P.PSTDKernel().initialize_kernel(tall)
     with pytest.raises(RuntimeError, match="receiver"):
-        ob.OracleScene(conf, 32)
+        ob.OracleScene(tall, 32)
 
 
 def test_receiver_outside_every_domain():
