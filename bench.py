@@ -1,32 +1,45 @@
 """Benchmark of the PSTD update loop on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lshape] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sources256] [--form gemm_f16]
 
-A step is one time step of the scene: 6 RK stages (fused line-derivative +
+A step is one time step of the scene(s): 6 RK stages (fused line-derivative +
 update launches), PML attenuation and receiver sampling, openPSTD's
-SingleThreadSolver::compute_timestep (kernel/Solver.cpp:71-93).  The default
-workload is the L-shaped three-domain scene (3 x 256x128 cells + PML, seed 0,
-2000 steps).  Prints one JSON line.
+SingleThreadSolver::compute_timestep (kernel/Solver.cpp:71-93).
+
+Default workload: one 1024 x 1024-cell geometry (two 512 x 1024 domains +
+50-cell PML) with 256 seeded source positions advanced together, 8 receivers
+each, in the GEMM form (dense derivative matrices on MFMA, FP16 hi+lo split
+operands, FP32 accumulation; the thin PML strips in the FFT form) -- the
+"GEMM+batched" side of the metric.  Prints one JSON line.
 
 value = cell-RK-stage updates/s = (cells of all non-rigid domains, PML
 included) * 6 * steps / wall time of the timed steps, state resident on the
-GPU.  roofline: algorithmic HBM bytes of one stage launch (48 B per cell:
-read p0, vx, vy of the previous stage and vx, vy, px, py at t_n, write vx,
-vy, px, py, p0) over the GPU time per stage launch, measured with a HIP event
-pair on the solver stream around the timed steps.  roofline.traffic: measured
-HBM bytes per stage launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the
-same counters' reading for a copy of known size), read from the committed
-profile summary of the workload when there is one.
+GPU.  The timed steps are issued eagerly with a HIP event pair on the solver
+stream around every launch of each kernel class (pstd_ctx_profile_kernels).
+roofline (GEMM form): algorithmic MFMA flops of one GEMM-kernel launch (3
+split products x 2 x K per line output) over that kernel's mean launch time
+from those events; peak 2.5 PFLOP/s dense FP16/BF16.  (FFT form: algorithmic
+HBM bytes of one stage launch, 48 B per cell, over the stage kernel's mean
+launch time; peak 8 TB/s.)  roofline.traffic: HBM bytes per launch of that
+kernel from the committed rocprofv3 counter summary of the workload (FETCH_SIZE
++ WRITE_SIZE scaled by the same counters' reading for a copy of known size),
+or null.
+plain_fft: the first N scenes advanced one at a time, each on its own solver in
+the plain FP32 FFT form (wall time of each scene's step loop, synchronised on
+both sides); vs_plain = value / that rate.
+accuracy: sources 0 and 1 of the workload advanced for the full 3000 steps in
+the benchmarked form against the committed FP64 oracle series
+(tests/golden/sources1024_receivers.npz): receiver rel-L2 and max-abs per
+source.
+secondary: the L-shaped three-domain scene in the FP32 FFT form (HBM roofline).
 cpu_baseline: the CPU oracle (FP32 restatement of the reference solver with
-OpenMP over calc tasks) on this host, over a bounded sample of the same scene.
---plain-compare N (batches): the first N scenes of the batch also run one at a
-time, each on its own solver in the FP32 FFT form (the plain path), and
-"plain_sequential" reports their cell-RK-stage/s (GPU time of the timed stage
-launches only, host gaps between scenes excluded) and batched / plain.
+OpenMP over calc tasks, openPSTD's MultiThreadSolver structure) on every core
+this process may use, over a bounded sample of the same workload.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -36,13 +49,13 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-RK-stage updates/sec (GEMM+batched vs plain FFT) at ≤1e-4 rel-L2 vs FP64"
 UNIT = "cell-RK-stage updates/s"
 HBM_PEAK_GBS = 8000.0
-MFMA_F16_PEAK_TFLOPS = 2500.0  # dense FP16 MFMA
+MFMA_PEAK_TFLOPS = 2500.0  # dense FP16 / BF16 MFMA
 BYTES_PER_CELL_STAGE = 48
 
-
-WORKLOADS = ["lshape", "room", "square4096", "batch64", "sources256"]
-DEFAULT_STEPS = {"lshape": 2000, "room": 500, "square4096": 100, "batch64": 100, "sources256": 10}
-DEFAULT_WARMUP = {"lshape": 50, "room": 50, "square4096": 10, "batch64": 10, "sources256": 3}
+WORKLOADS = ["sources256", "lshape", "room", "square4096", "batch64"]
+DEFAULT_STEPS = {"lshape": 2000, "room": 500, "square4096": 100, "batch64": 100, "sources256": 20}
+DEFAULT_WARMUP = {"lshape": 50, "room": 50, "square4096": 10, "batch64": 10, "sources256": 5}
+FORM_DTYPE = {"fft": "f32", "gemm_f16": "f16x2-split/f32-accumulate", "gemm_bf16": "bf16x2-split/f32-accumulate"}
 
 
 def workload(name, steps):
@@ -67,25 +80,92 @@ def workload(name, steps):
     raise SystemExit(f"unknown workload {name}")
 
 
-def plain_sequential(confs, n, steps):
-    """The first n scenes advanced one at a time (one solver each, FP32 FFT
-    form, autotuned like the batch): summed cell-stages over summed GPU time of
-    their timed stage launches."""
+def make_solver(confs, form):
     import openpstd_amd as P
-    gpu_s, cell_stages = 0.0, 0
+    if len(confs) == 1:
+        kernel = P.PSTDKernel(GPU=True)
+        kernel.initialize_kernel(confs[0])
+        kernel.set_form(form)
+        return kernel.prepare(), [kernel]
+    dev = P.BatchSolver(confs, form=form)
+    return dev, dev.kernels
+
+
+def plain_fft(confs, n, steps, warmup=2):
+    """The first n scenes advanced one at a time (one solver each, FP32 FFT
+    form, autotuned like the batch): summed cell-stages over the summed wall
+    time of their step loops (and over the GPU time of their stage launches)."""
+    import openpstd_amd as P
+    wall, gpu_s, cell_stages = 0.0, 0.0, 0
     for conf in confs[:n]:
         k = P.PSTDKernel(GPU=True)
         k.initialize_kernel(conf)
         k.set_form("fft")
         dev = k.prepare()
-        dev.advance_async(2)
+        dev.advance_async(warmup)
         dev.synchronize()
+        t0 = time.perf_counter()
+        dev.advance_async(steps)
+        dev.synchronize()
+        wall += time.perf_counter() - t0
         ms, launches = dev.profile_stages(steps)
         gpu_s += ms * 1e-3 * launches
-        cell_stages += dev.cells_per_stage * launches
+        cell_stages += dev.cells_per_stage * 6 * steps
         del dev, k
-    return {"value": cell_stages / gpu_s, "unit": UNIT, "scenes": min(n, len(confs)), "steps_per_scene": steps,
-            "form": "fft"}
+    return {"value": cell_stages / wall, "unit": UNIT, "value_gpu_time": cell_stages / gpu_s,
+            "scenes": min(n, len(confs)), "steps_per_scene": steps, "form": "fft",
+            "method": "each scene on its own solver, FP32 FFT form; cell-stages over the wall time of its "
+                      "step loop (advance + synchronise); value_gpu_time over the GPU time of its stage launches"}
+
+
+def accuracy(form, steps=3000):
+    """Sources 0 and 1 of the 256-source workload, full 3000 steps, against the
+    committed FP64 oracle series: {source: (rel-L2, max-abs)}."""
+    import numpy as np
+    import openpstd_amd as P
+    from openpstd_amd import workloads as W
+    path = os.path.join(ROOT, "tests", "golden", "sources1024_receivers.npz")
+    if not os.path.exists(path):
+        return None
+    g = np.load(path)
+    confs = W.source_batch(0, steps, n_sources=2)
+    b = P.BatchSolver(confs, form=form)
+    got = b.advance(steps)
+    out = {}
+    for i in range(2):
+        s, want = b.scene_samples(got, i), g[f"fp64_source{i}"]
+        out[f"source{i}"] = {"rel_l2": float(np.linalg.norm(s - want) / np.linalg.norm(want)),
+                             "max_abs": float(np.abs(s - want).max())}
+    return {"reference": "FP64 oracle, tests/golden/sources1024_receivers.npz", "steps": steps, "form": form,
+            "per_source": out, "max_rel_l2": max(v["rel_l2"] for v in out.values())}
+
+
+def measured_traffic(name, kernel):
+    """HBM bytes per launch of `kernel` from profiles/<kernel>_<workload>_traffic.md
+    (line 'traffic_bytes_per_launch: N'), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{name}_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
+def secondary_lshape(steps=400, warmup=50):
+    """The L-shaped scene in the FP32 FFT form: cell-stage/s and HBM roofline."""
+    dev, _ = make_solver(workload("lshape", steps + warmup)[0], "fft")
+    cells = dev.cells_per_stage
+    dev.advance_async(warmup)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    mean_ms, launches = dev.profile_stages(steps)
+    wall = time.perf_counter() - t0
+    achieved = BYTES_PER_CELL_STAGE * cells / (mean_ms * 1e-3) / 1e9
+    return {"workload": workload("lshape", 1)[1], "form": "fft", "value": cells * 6 * steps / wall, "unit": UNIT,
+            "steps": steps, "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                         "frac": achieved / HBM_PEAK_GBS, "mean_launch_us": mean_ms * 1e3,
+                                         "note": "about 3 workgroups per CU: launch- and latency-limited"}}
 
 
 def _oracle_rate(confs, threads, seconds, max_steps):
@@ -117,28 +197,31 @@ def _oracle_rate(confs, threads, seconds, max_steps):
     return cell_steps * 6 / elapsed, f"{what} ({elapsed:.1f} s timed), FP32 oracle, {threads} OpenMP thread(s)"
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(confs, seconds=10.0, max_steps=400):
-    """FP32 oracle on the host cores: MultiThreadSolver-style OpenMP over the
-    calc tasks (the reported value), and single-threaded as
-    SingleThreadSolver, each timed over a bounded sample of the same
-    workload."""
-    threads = min(16, os.cpu_count() or 1)
+    """FP32 oracle on all host cores available to this process:
+    MultiThreadSolver-style OpenMP over the calc tasks (the reported value), and
+    single-threaded as SingleThreadSolver, each timed over a bounded sample of
+    the same workload."""
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
     value, sample = _oracle_rate(confs, threads, seconds, max_steps)
     single, single_sample = _oracle_rate(confs, 1, seconds / 2, max_steps)
     return {"value": value, "unit": UNIT, "cores": threads, "kind": "port", "sample": sample,
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "fft": "the oracle's own radix-2 FFT (FFTW is not available)",
             "single_thread": {"value": single, "cores": 1, "sample": single_sample}}
-
-
-def measured_traffic(name):
-    """HBM bytes per stage launch from profiles/stage_kernel_<workload>_traffic.md
-    (line 'traffic_bytes_per_launch: N'), or None."""
-    path = os.path.join(ROOT, "profiles", f"stage_kernel_{name}_traffic.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
 
 
 def copy_calibration(mib):
@@ -159,19 +242,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default depends on the workload)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default depends on the workload)")
-    ap.add_argument("--workload", default="lshape", choices=WORKLOADS)
-    ap.add_argument("--graph-steps", type=int, default=0, help="replay blocks of this many steps as a hipGraph")
-    ap.add_argument("--form", default="fft", choices=["fft", "gemm"],
-                    help="line derivative form: FP32 FFT, or dense matrices on MFMA (FP16 hi+lo split)")
+    ap.add_argument("--workload", default="sources256", choices=WORKLOADS)
+    ap.add_argument("--form", default="gemm_f16", choices=["fft", "gemm_f16", "gemm_bf16"],
+                    help="line derivative form: FP32 FFT, or dense matrices on MFMA (FP16 / BF16 hi+lo split)")
+    ap.add_argument("--graph-steps", type=int, default=None,
+                    help="replay blocks of this many steps as a hipGraph (default 20; kernel-class timing runs "
+                         "eagerly)")
     ap.add_argument("--tile-order", default="auto", choices=["auto", "xcd", "plain"],
                     help="workgroup-to-tile order (default: the autotuner's choice)")
     ap.add_argument("--flavour", default="auto", choices=["auto", "early", "late", "mid"],
-                    help="stage-kernel flavour (default: the autotuner's choice)")
+                    help="FFT stage-kernel flavour (default: the autotuner's choice)")
     ap.add_argument("--tile-threads", default="auto", choices=["auto", "128", "256", "512"],
                     help="block size for lines of FFT length <= 2048 (default: the autotuner's choice)")
+    ap.add_argument("--plain-compare", type=int, default=8, metavar="N",
+                    help="batches: time the first N scenes one at a time in the plain FFT form (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--plain-compare", type=int, default=0, metavar="N",
-                    help="batches: also time the first N scenes one at a time in the plain FFT form")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a copy of MIB MiB (for scaling HBM byte counters)")
     args = ap.parse_args()
@@ -183,21 +270,11 @@ def main():
         copy_calibration(args.copy_calibration)
         return
 
-    import openpstd_amd as P
-
     steps = args.steps if args.steps is not None else DEFAULT_STEPS[args.workload]
     warmup = args.warmup if args.warmup is not None else DEFAULT_WARMUP[args.workload]
-    confs, desc = workload(args.workload, warmup + steps)
-    if len(confs) == 1:
-        kernel = P.PSTDKernel(GPU=True)
-        kernel.initialize_kernel(confs[0])
-        kernel.set_form(args.form)
-        kernels = [kernel]
-        dev = kernel.prepare()
-    else:
-        dev = P.BatchSolver(confs, form=args.form)
-        kernels = dev.kernels
-    dev.set_graph_steps(args.graph_steps)
+    confs, desc = workload(args.workload, warmup + 3 * steps + 4)
+    dev, kernels = make_solver(confs, args.form)
+    dev.set_graph_steps(20 if args.graph_steps is None else args.graph_steps)
     if args.tile_order != "auto":
         dev.set_tile_order(args.tile_order == "xcd")
     if args.flavour != "auto":
@@ -210,27 +287,39 @@ def main():
     dev.advance_async(warmup)
     dev.synchronize()
     t0 = time.perf_counter()
-    # HIP events on the solver stream around the timed steps; returns after they finished
-    mean_ms, launches = dev.profile_stages(steps)
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    prof = dev.profile_kernels(steps)  # timed steps; returns after they finished
+    wall = time.perf_counter() - t0
     value = cells * 6 * steps / wall
-    achieved = BYTES_PER_CELL_STAGE * cells / (mean_ms * 1e-3) / 1e9
-    traffic = measured_traffic(args.workload)
+    stage_ms = prof["total"][0] / prof["total"][1]
 
     if args.form == "fft":
+        cls = max(("fft_small", "fft_4096", "fft_8192"), key=lambda c: prof[c][0])
+        mean_ms = prof[cls][0] / prof[cls][1]
+        kname = {"fft_small": "stage_small_kernel", "fft_4096": "stage_big_kernel_4096",
+                 "fft_8192": "stage_big_kernel_8192"}[cls]
+        achieved = BYTES_PER_CELL_STAGE * cells / (stage_ms * 1e-3) / 1e9
+        traffic, tsrc = measured_traffic(args.workload, kname)
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                    "kernel": "stage kernel (one launch per RK stage)", "mean_launch_us": mean_ms * 1e3,
-                    "launches_timed": launches, "bytes_per_launch": BYTES_PER_CELL_STAGE * cells}
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
+                    "kernel": f"the stage's FFT-form launches (dominant: {kname}, mean launch "
+                              f"{mean_ms * 1e3:.1f} us); 48 B per cell of every non-rigid domain over the stage time",
+                    "mean_launch_us": mean_ms * 1e3, "stage_us": stage_ms * 1e3,
+                    "bytes_per_stage": BYTES_PER_CELL_STAGE * cells}
     else:
         flops, outs = dev.gemm_work()
+        g_ms, g_n = prof["gemm"]
+        mean_ms = g_ms / g_n
         tflops = flops / (mean_ms * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": tflops, "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": tflops / MFMA_F16_PEAK_TFLOPS, "traffic": None,
-                    "kernel": "GEMM-form stage launches (MFMA flops of the GEMM segments over the whole stage time)",
-                    "mean_launch_us": mean_ms * 1e3, "launches_timed": launches, "flops_per_stage": flops,
-                    "gemm_line_outputs_per_stage": outs}
+        traffic, tsrc = measured_traffic(args.workload, "stage_gemm_kernel")
+        roofline = {"bound": "mfma", "achieved": tflops, "peak": MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": tflops / MFMA_PEAK_TFLOPS, "traffic": traffic, "traffic_source": tsrc,
+                    "kernel": "stage_gemm_kernel (one launch per RK stage: every GEMM-form tile of every scene)",
+                    "mean_launch_us": mean_ms * 1e3, "launches_timed": g_n, "flops_per_launch": flops,
+                    "gemm_line_outputs_per_launch": outs,
+                    "whole_stage": {"stage_us": stage_ms * 1e3,
+                                    "achieved_tflops": flops / (stage_ms * 1e-3) / 1e12,
+                                    "frac": flops / (stage_ms * 1e-3) / 1e12 / MFMA_PEAK_TFLOPS}}
+    kernel_times = {c: {"ms_total": v[0], "launches": v[1]} for c, v in prof.items() if v[1]}
     infos = [k.domain_info(i) for k in kernels for i in range(k.domain_count())]
     user = sum(d["nx"] * d["ny"] for d in infos if not d["is_pml"])
     out = {
@@ -244,23 +333,30 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.form == "fft" else "f16x2-split/f32-accumulate",
+        "dtype": FORM_DTYPE[args.form],
         "data": "synthetic",
         "config": {"workload": desc, "scenes": len(confs), "cells_per_stage": cells, "user_cells": user,
-                   "domains": len(infos), "rk_stages_per_step": 6, "graph_steps": args.graph_steps,
-                   "launch_shape": {"flavour": ["early", "late", "mid"][tuning["lean"]], "xcd_order": tuning["xcd_order"],
-                                    "tile_threads": tuning["tile_threads"],
+                   "domains": len(infos), "rk_stages_per_step": 6, "form": args.form,
+                   "launch_shape": {"flavour": ["early", "late", "mid"][tuning["lean"]],
+                                    "xcd_order": tuning["xcd_order"], "tile_threads": tuning["tile_threads"],
                                     "autotuned": args.tile_order == "auto" and args.flavour == "auto"
                                     and args.tile_threads == "auto"},
                    "path": ("FP32 FFT (fused LDS line derivative + RK update)" if args.form == "fft" else
-                            "dense derivative matrices on MFMA, FP16 hi+lo split operands, FP32 accumulate "
-                            "(short segments in the FP32 FFT form)")},
+                            "dense derivative matrices on MFMA, 16-bit hi+lo split operands staged from FP32 "
+                            "in the kernel, FP32 accumulate, fused RK update (thin PML strips in the FP32 FFT "
+                            "form)")},
         "roofline": roofline,
+        "kernel_times": kernel_times,
     }
+    del dev
     if args.plain_compare and len(confs) > 1:
-        plain = plain_sequential(confs, args.plain_compare, min(steps, 20))
-        plain["batched_over_plain"] = value / plain["value"]
-        out["plain_sequential"] = plain
+        plain = plain_fft(confs, args.plain_compare, steps)
+        out["plain_fft"] = plain
+        out["vs_plain"] = value / plain["value"]
+    if args.workload == "sources256" and not args.no_accuracy:
+        out["accuracy"] = accuracy(args.form)
+    if not args.no_secondary and args.workload != "lshape":
+        out["secondary"] = secondary_lshape()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(confs)
     print(json.dumps(out))

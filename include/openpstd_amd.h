@@ -75,9 +75,12 @@ typedef struct pstd_domain_conf {
  * p0(rel.x, rel.y) with p0 stored (row = y, col = x)
  * (kernel/core/Receiver.cpp:80-83), i.e. the transposed cell.
  * PSTD_RECEIVER_PHYSICAL (default) samples p0[rel.y][rel.x];
- * PSTD_RECEIVER_REFERENCE samples row rel.x, column rel.y as openPSTD does and
- * rejects receivers for which that cell is outside the domain (openPSTD's
- * Eigen bounds assertion). */
+ * PSTD_RECEIVER_REFERENCE reads what openPSTD's p0(rel.x, rel.y) reads from
+ * its column-major ny x nx array in an NDEBUG build: storage element
+ * rel.x + rel.y * ny, i.e. the cell (row (rel.x + rel.y*ny) % ny, column
+ * (rel.x + rel.y*ny) / ny) -- the transposed cell when rel.x < ny -- and
+ * rejects receivers whose element lies past the array (a build with
+ * assertions aborts on every rel.x >= ny). */
 #define PSTD_RECEIVER_PHYSICAL 0
 #define PSTD_RECEIVER_REFERENCE 1
 
@@ -163,13 +166,18 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan);
 /* Derivative form (pstd_ctx_set_form, before the plan is uploaded).
  * PSTD_FORM_FFT: every line derivative as openPSTD computes it (windowed line,
  *   FFT, spectral factor, inverse FFT) in FP32.
- * PSTD_FORM_GEMM_F16: segments whose lines and outputs fill 128 x 128 blocks
- *   at least to 3/4 apply the segment's dense derivative matrix
- *   (pstd_derivative_matrix) on MFMA with FP16 hi + lo split operands (inputs
- *   scaled per line by a power of two) and FP32 accumulation; the other
- *   segments (thin PML strips) keep the FFT form. */
+ * PSTD_FORM_GEMM_F16: segments whose lines and outputs fill 128-line x
+ *   256-output tiles at least to 3/4 apply the segment's dense derivative
+ *   matrix (pstd_derivative_matrix) on MFMA with FP16 hi + lo split operands
+ *   (22-bit significands; the inputs scaled per scene by a power of two, from
+ *   the largest value the previous RK stage wrote) and FP32 accumulation; the
+ *   last output of a pressure line (n + 1 outputs) is an FP32 dot product;
+ *   the other segments (thin PML strips) keep the FFT form.
+ * PSTD_FORM_GEMM_BF16: the same with BF16 hi + lo operands (16-bit
+ *   significands, no range limit). */
 #define PSTD_FORM_FFT 0
 #define PSTD_FORM_GEMM_F16 1
+#define PSTD_FORM_GEMM_BF16 2
 int pstd_ctx_set_form(pstd_ctx *ctx, int form);
 /* Uploads several independent scenes as one plan: every RK stage of all of
  * them is one launch.  Domains and receivers are numbered across the scenes in
@@ -199,6 +207,15 @@ int pstd_ctx_read_frame(pstd_ctx *ctx, int domain, float *out, size_t count);
  * launch count.  For measurement; the state advances exactly as with
  * pstd_ctx_advance (samples stay on the device). */
 int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int *launches);
+/* Advances `steps` time steps like pstd_ctx_advance_async, issuing every
+ * launch eagerly with a HIP event pair on the solver stream around each
+ * launch of a kernel class, and waits.  ms[c] / launches[c] receive the summed
+ * GPU time and launch count of class c: 0 the FFT-form kernel of lines up to
+ * 2048 points (with the passthrough and receiver tiles), 1 the 4096-point
+ * kernel, 2 the 8192-point kernel, 3 the GEMM-form kernel; ms[4] the whole
+ * run, launches[4] its 6*steps RK stages.  For measurement (no counterpart in
+ * the reference). */
+int pstd_ctx_profile_kernels(pstd_ctx *ctx, int steps, double ms[5], int launches[5]);
 /* Replays whole blocks of `steps` time steps as one captured hipGraph (0 = the
  * stage launches are issued one by one, the default).  The graph is captured
  * on first use after the first time step and dropped on re-upload.  Results

@@ -19,7 +19,9 @@ PSTD_RECEIVER_REFERENCE = 1
 
 PSTD_FORM_FFT = 0
 PSTD_FORM_GEMM_F16 = 1
-FORMS = {"fft": PSTD_FORM_FFT, "gemm": PSTD_FORM_GEMM_F16}
+PSTD_FORM_GEMM_BF16 = 2
+FORMS = {"fft": PSTD_FORM_FFT, "gemm": PSTD_FORM_GEMM_F16, "gemm_f16": PSTD_FORM_GEMM_F16,
+         "gemm_bf16": PSTD_FORM_GEMM_BF16}
 
 
 class pstd_settings(C.Structure):

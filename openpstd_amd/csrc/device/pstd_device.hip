@@ -64,7 +64,7 @@ struct DomainDev {
 //  kind 1: passthrough — elements [line0, line0+count) of one field that no
 //          segment updates (rigid domains, directions openPSTD skips).
 //  kind 2: receiver sampling (first stage of a step).
-//  kind 3: the segment addressing of a GEMM-form line block (GemmDesc::t).
+//  kind 3: the segment addressing of a GEMM-form tile (GemmTile::t).
 // Memory is addressed through buffer resources over one domain block (base
 // `*_dom`, `*_size` elements): value (line, pos) of a source field is element
 // a + line*ls + pos*ps of the block (+ element b of the second array for
@@ -85,6 +85,7 @@ struct TileDesc {
     int o_f, o_ls, o_ps;  // output field (vx, px, vy or py) within the main block
     float coef;           // 1/rho (velocity update) or rho*c1^2 (pressure update)
     int is_pml;
+    int scene;            // scene of the batch (per-scene maxima of the GEMM form's scaling)
 };
 
 struct ReceiverCell {
@@ -110,6 +111,11 @@ struct StageArgs {
     float *samples;    // samples[slot * nrec + r], slot < 0: no sampling in this launch
     int slot;
     int remap;         // XCD-contiguous tile order (tile_index)
+    // GEMM form (umax == nullptr otherwise): per-scene maxima of the written
+    // values in three rotating slots (write slot_w, read slot_r, clear slot_z)
+    unsigned *umax;
+    int slot_w, slot_r, slot_z, n_scenes;
+    const void *dmat;  // derivative matrices (2-byte elements)
 };
 
 // ------------------------------------------------------------- device code
@@ -417,6 +423,8 @@ __device__ __forceinline__ void prefetch_twiddles(float2 *twr, int t, const floa
     fft_load_twiddles<F>(twr, t, tw);
 }
 
+__device__ __forceinline__ void track_max(const StageArgs &a, int scene, int cls, float m);
+
 // Solver::update_field_values (kernel/Solver.cpp:174-181): out = t_n - s*L*coef,
 // the last stage also t_n = out * PML factor.  Positions outside [0, n_out)
 // and lines past the tile get an out-of-block offset (no store).
@@ -471,6 +479,15 @@ __device__ __forceinline__ void finish_updates(const StageArgs &a, const TileDes
     for (int m = 0; m < 8; ++m) {
         nv[m][0] = __builtin_fmaf(-sc, v[m].x, u.tnv[m][0]);
         nv[m][1] = __builtin_fmaf(-sc, v[m].y, u.tnv[m][1]);
+    }
+    if (a.umax) {
+        float mx = 0.f;
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+                if (u.off[m][q] != kOOB) mx = fmaxf(mx, fabsf(nv[m][q]));
+        track_max(a, td.scene, td.vel ? 0 : 1, mx);
     }
     if (LAYOUT == 1 && 2 * p + 1 < td.count) {
         // columns, both lines in the tile: 8-byte stores
@@ -594,6 +611,12 @@ __device__ void passthrough_tile(const StageArgs &a, const TileDesc &td) {
         bstore(ro, off[k], v[k]);
         if (use_pml) bstore(rt, off[k], v[k] * f[k]);
     }
+    if (a.umax) {
+        float mx = 0.f;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) mx = fmaxf(mx, fabsf(v[k]));
+        track_max(a, td.scene, td.vel ? 0 : 1, mx);  // vel: 1 for px/py (as the velocity lines that write them)
+    }
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on XCD
@@ -662,288 +685,488 @@ __global__ void p0_materialise_kernel(const float *__restrict__ src, float *__re
 
 // ------------------------------------------------------------- GEMM form
 // The windowed line derivative of a segment is a fixed linear operator: the
-// dense matrix D (n_out x K) of plan_tables.h.  Per RK stage, for every block
-// of up to 128 lines of a GEMM-form segment:
-//  1. stage_umax_kernel and stage_usplit_kernel gather the raw input lines
-//     U = [side1 tail | main | side2 head] (K values each), scale every line
-//     by the power of two that puts its largest value just below 2^15, and
-//     store U as FP16 hi + lo parts (U = hi + lo to 22 bits) in k-contiguous
-//     rows;
-//  2. stage_gemm_kernel computes 128 lines x 128 outputs of C = U D^T on MFMA
-//     (v_mfma_f32_32x32x16_f16) as U_hi D_hi + U_hi D_lo + U_lo D_hi with FP32
-//     accumulation (D split the same way on the host, one power of two per
-//     matrix), undoes both scalings and applies the RK update as the FFT form
-//     does.  X lines put U on the MFMA A side (rows of C = lines); Y lines put
-//     D there (rows = outputs), so that the C columns consecutive lanes hold
-//     are consecutive in memory in both directions.
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+// dense matrix D (n_out x K) of plan_tables.h, applied to the raw line
+// u = [side1 tail | main | side2 head] (K values).  One workgroup computes
+// 128 lines x 256 outputs of C = U D^T on MFMA and applies the RK update:
+//  * the input lines are read as FP32 straight from the stage state (p0 as
+//    px + py), scaled by the scene's power of two 2^e and split into two
+//    16-bit parts while they are staged into LDS (U = hi + lo: FP16 keeps 22
+//    bits when every scaled value is below 2^15, BF16 16 bits at any scale);
+//  * D is stored split the same way (one power of two per matrix);
+//  * C = U_hi D_hi + U_hi D_lo + U_lo D_hi with v_mfma_f32_32x32x16_{f16,bf16},
+//    FP32 accumulation, over k-chunks of 32 through a double-buffered LDS
+//    ring (one workgroup barrier per chunk, the global loads of chunk c+2 in
+//    flight while chunk c is multiplied);
+//  * the scene's scale comes from the largest |value| every kernel of the
+//    previous stage wrote (a per-scene atomic maximum in three rotating
+//    slots: stage k reads slot (k+2)%3, writes slot k%3 and clears slot
+//    (k+1)%3), so no pass over the inputs precedes the multiplication;
+//  * pressure lines have n+1 outputs: the multiplication covers the first n,
+//    the last is an FP32 dot product of the raw line with D's last row,
+//    accumulated by the threads while they stage the line (its 256-wide
+//    output tiles then fit n exactly instead of wasting most of a tile).
+// X lines put U on the MFMA A side (rows of C = lines, columns = outputs),
+// Y lines put D there (rows = outputs, columns = lines), so that consecutive
+// lanes store consecutive floats in both directions.
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kGemmThreads = 256;
-constexpr int kGB = 128;        // lines per line block, outputs per GEMM block
-constexpr int kCK = 32;         // k per chunk of the split pass
-constexpr int kGK = 64;         // k per LDS chunk of the GEMM (four MFMA k-steps)
-constexpr int kGKP = kGK + 8;   // GEMM LDS row pitch in halfs (144 B)
-
-struct GemmDesc {
-    TileDesc t;            // segment addressing; line0/count: the line block (count <= 128)
-    long long d_hi, d_lo;  // D parts (halfs), row-major [n_pad][k_pad]
-    long long u_hi, u_lo;  // the line block's split inputs (halfs), row-major [128][k_pad]
-    long long e_off;       // the line block's 128 scale exponents
-    int k_pad, n0, d_exp, K;
+template <typename E>
+struct MfmaOps;
+template <>
+struct MfmaOps<_Float16> {
+    typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+    typedef _Float16 v4 __attribute__((ext_vector_type(4)));
+    __device__ static f32x16 mfma(v8 a, v8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct MfmaOps<__bf16> {
+    typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+    typedef __bf16 v4 __attribute__((ext_vector_type(4)));
+    __device__ static f32x16 mfma(v8 a, v8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
 };
 
-struct GemmArgs {
-    StageArgs st;
-    const GemmDesc *__restrict__ tiles;
-    const _Float16 *dmat;
-    _Float16 *umat;
-    int *uexp;
+constexpr int kGemmThreads = 512;  // 8 waves: 2 line groups x 4 output groups of 64 x 64
+constexpr int kGL = 128;           // lines per tile
+constexpr int kGN = 256;           // outputs per tile
+constexpr int kGK = 32;            // k per LDS chunk (two MFMA k-steps)
+constexpr int kGP = kGK + 8;       // LDS row pitch in 2-byte elements (80 B: conflict-free 16-byte operand reads)
+constexpr int kGemmLdsU = 2 * 2 * kGL * kGP;  // elements: [buffer][part][line][k]
+constexpr int kGemmLdsD = 2 * 2 * kGN * kGP;  // elements: [buffer][part][output][k]
+constexpr size_t kGemmLdsBytes = (size_t)(kGemmLdsU + kGemmLdsD) * 2;
+
+struct GemmTile {
+    TileDesc t;            // segment addressing; line0/count: the tile's lines (count <= 128)
+    long long d_hi, d_lo;  // D parts (2-byte elements), row-major [n_rows][k_pad]
+    int n_rows, k_pad;     // stored rows (multiple of 32) and columns (multiple of kGK) of D
+    int n0;                // first output of the tile
+    int n_g;               // outputs the multiplication produces (n_out, or n_out - 1 with the dot row)
+    int d_exp;             // D was stored as D * 2^d_exp
+    int K;                 // line length
+    int dot_off;           // >= 0: ftab offset of D's last row (FP32): this tile writes output n_out - 1
 };
 
-// Element (line, k) of the line block for 16 (line, k) pairs of one chunk of
-// kCK k: rows (dir 0) pass i covers lines 8i .. 8i+7 with lanes along k,
-// columns (dir 1) covers 128 lines x 2 k with lanes along the lines.
-__device__ __forceinline__ int split_line(const TileDesc &td, int i) {
-    return td.dir == 0 ? i * 8 + ((int)threadIdx.x >> 5) : ((int)threadIdx.x & 127);
-}
-__device__ __forceinline__ int split_k(const TileDesc &td, int i) {
-    return td.dir == 0 ? ((int)threadIdx.x & 31) : 2 * i + ((int)threadIdx.x >> 7);
-}
-
-__device__ __forceinline__ void split_load_chunk(const StageArgs &a, const TileDesc &td, int K, int k0,
-                                                 float (&val)[16]) {
-    const int w = td.wlen, nm = td.n_main, vel = td.vel;
-    const float *A = vel ? a.vsrc : a.p0a;
-    const bool hasB = !vel && a.p0b != nullptr;
-    const float *B = hasB ? a.p0b : A;
-    const bool ov1 = k0 < w && td.s1_len >= 0, ovm = k0 < w + nm && k0 + kCK > w,
-               ov2 = k0 + kCK > w + nm && k0 < K && td.s2_len >= 0;
+// Per-scene maxima of |value| written in a stage: umax[(slot * n_scenes + scene) * 2 + class],
+// class 0 the pressure fields (px, py), class 1 the velocities; float bits.
+__device__ __forceinline__ void track_max(const StageArgs &a, int scene, int cls, float m) {
+    if (!a.umax) return;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) val[i] = 0.f;
-    if (ovm) {
-        const Rsrc rA = block_rsrc(A, td.m_dom, td.m_size);
-        const Rsrc rB = block_rsrc(B, td.m_dom + (td.m_b - td.m_a), td.m_size - (td.m_b - td.m_a));
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int line = split_line(td, i), k = k0 + split_k(td, i);
-            const bool ok = line < td.count && k >= w && k < w + nm;
-            const int off = ok ? (td.m_a + (td.line0 + line) * td.m_ls + (k - w) * td.m_ps) * 4 : kOOB;
-            val[i] += hasB ? bload(rA, off) + bload(rB, off) : bload(rA, off);
-        }
-    }
-    if (ov1) {
-        const Rsrc rA = block_rsrc(A, td.s1_dom, td.s1_size);
-        const Rsrc rB = block_rsrc(B, td.s1_dom + (td.s1_b - td.s1_a), td.s1_size - (td.s1_b - td.s1_a));
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int line = split_line(td, i), k = k0 + split_k(td, i);
-            const bool ok = line < td.count && k < w;
-            const int off =
-                ok ? (td.s1_a + (td.line0 + line + td.s1_line) * td.s1_ls + (td.s1_len - w - vel + k) * td.s1_ps) * 4
-                   : kOOB;
-            val[i] += hasB ? bload(rA, off) + bload(rB, off) : bload(rA, off);
-        }
-    }
-    if (ov2) {
-        const Rsrc rA = block_rsrc(A, td.s2_dom, td.s2_size);
-        const Rsrc rB = block_rsrc(B, td.s2_dom + (td.s2_b - td.s2_a), td.s2_size - (td.s2_b - td.s2_a));
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int line = split_line(td, i), k = k0 + split_k(td, i);
-            const bool ok = line < td.count && k >= w + nm && k < K;
-            const int off =
-                ok ? (td.s2_a + (td.line0 + line + td.s2_line) * td.s2_ls + (k - w - nm + vel) * td.s2_ps) * 4 : kOOB;
-            val[i] += hasB ? bload(rA, off) + bload(rB, off) : bload(rA, off);
-        }
-    }
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0 && m > 0.f)
+        atomicMax(&a.umax[((size_t)a.slot_w * a.n_scenes + scene) * 2 + cls], __float_as_uint(m));
 }
 
-// The split pass runs as two launches over pieces of 128 lines x 64 k (a
-// piece loads its 8192 values at once): stage_umax_kernel takes the largest
-// |u| of every line (an atomic maximum over the pieces of a line block into
-// one of two slot sets, which alternate between stages: the k = 0 pieces clear
-// the other set for the next stage), stage_usplit_kernel scales and splits.
-struct SplitPiece {
-    int lb;  // line block (index of its descriptor)
-    int k0;  // first k of the piece
+// Power of two that brings a line whose values are bounded by m below 2^15
+// (FP16 keeps both parts normal down to 2^-3 of that bound).
+__device__ __forceinline__ int gemm_scale_exp(const StageArgs &a, int scene, int vel) {
+    const unsigned bits = a.umax[((size_t)a.slot_r * a.n_scenes + scene) * 2 + (vel ? 1 : 0)];
+    const float m = __uint_as_float(bits) * (vel ? 1.f : 2.f);  // p0 = px + py
+    return m > 0.f ? min(max(14 - ilogbf(m), -100), 100) : 0;
+}
+
+// Raw input values of one chunk held by a thread: 8 consecutive k of one line
+// (X lines: 4 threads per line, so 4 lanes read 128 contiguous bytes of a row;
+// Y lines: one line per lane, so a wavefront reads 64 adjacent columns of a
+// row).  The second array of p0 = px + py is kept apart and added when the
+// chunk is converted, so that nothing waits for the loads of chunk c+2 before
+// the MFMAs of chunk c; dk: the FP32 dot row's coefficients of these k.
+struct UChunk {
+    float a[8], b[8], dk[8];
+};
+struct DChunk {
+    u32x4 v[4];
 };
 
-__global__ void __launch_bounds__(kGemmThreads) stage_umax_kernel(GemmArgs g, const SplitPiece *__restrict__ pieces,
-                                                                  int *umax, long long n_slots, int parity) {
-    __shared__ int emax[kGB];
-    const SplitPiece pc = pieces[blockIdx.x];
-    const GemmDesc gd = g.tiles[pc.lb];
-    const TileDesc &td = gd.t;
+template <int DIR>
+__device__ __forceinline__ int stage_line(int tid) {
+    return DIR == 0 ? tid >> 2 : tid & (kGL - 1);
+}
+template <int DIR>
+__device__ __forceinline__ int stage_kq(int tid) {
+    // Y: the k-group is that of the whole wavefront
+    return DIR == 0 ? (tid & 3) * 8 : __builtin_amdgcn_readfirstlane(tid >> 7) * 8;
+}
+
+// raw buffer over `bytes` bytes at byte offset `byte_off` of `base`
+__device__ __forceinline__ Rsrc bytes_rsrc(const void *base, long long byte_off, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)((const char *)base + byte_off), (short)0, bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float4 bload4(Rsrc r, int off_bytes) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off_bytes, 0, 0));
+}
+
+// Line `line` (absolute) of a tile: byte offsets of position k of its raw
+// input line in the three blocks it comes from (main, side1, side2).
+struct LineAddr {
+    int w, nm, vel, K;
+    int m_base, m_ps, s1_base, s1_ps, s2_base, s2_ps;  // offset of k = 0 of each piece, byte step per k
+    bool s1, s2;
+    __device__ LineAddr(const TileDesc &td, int line, int K_) {
+        w = td.wlen;
+        nm = td.n_main;
+        vel = td.vel;
+        K = K_;
+        m_ps = td.m_ps * 4;
+        m_base = (td.m_a + line * td.m_ls) * 4 - w * m_ps;
+        s1 = td.s1_len >= 0;
+        s2 = td.s2_len >= 0;
+        s1_base = (td.s1_a + (line + td.s1_line) * td.s1_ls + (td.s1_len - w - vel) * td.s1_ps) * 4;
+        s1_ps = td.s1_ps * 4;
+        s2_base = (td.s2_a + (line + td.s2_line) * td.s2_ls + (vel - w - nm) * td.s2_ps) * 4;
+        s2_ps = td.s2_ps * 4;
+    }
+    __device__ int main_off(int k) const { return k >= w && k < w + nm ? m_base + k * m_ps : kOOB; }
+    __device__ int s1_off(int k) const { return s1 && k < w ? s1_base + k * s1_ps : kOOB; }
+    __device__ int s2_off(int k) const { return s2 && k >= w + nm && k < K ? s2_base + k * s2_ps : kOOB; }
+};
+
+// The arrays a tile's lines are read from: A (vx / vy for velocity lines, px
+// or the explicit p0 for pressure lines) and, for p0 = px + py, B (py).  The
+// three pieces of a line differ only in the domain block.
+struct GemmSrc {
+    const float *A, *B;
+    bool hasB;
+};
+
+__device__ __forceinline__ GemmSrc gemm_src(const StageArgs &a, const TileDesc &td) {
+    GemmSrc r;
+    r.A = td.vel ? a.vsrc : a.p0a;
+    r.hasB = !td.vel && a.p0b != nullptr;
+    r.B = r.hasB ? a.p0b : r.A;
+    return r;
+}
+
+// Which piece of the line a whole chunk [k0, k0 + kGK) lies in (the same for
+// every line of a tile).  kPieceS2 chunks may run past K: positions from K on
+// read zero (a value from past the halo could be another field, outside the
+// range the scene's scale was chosen for).
+constexpr int kPieceMain = 0, kPieceS1 = 1, kPieceS2 = 2, kPieceMixed = 3, kPieceNone = 4;
+__device__ __forceinline__ int chunk_piece(const TileDesc &td, int K, int k0) {
+    const int w = td.wlen, nm = td.n_main;
+    return k0 >= K                              ? kPieceNone
+           : k0 >= w && k0 + kGK <= w + nm      ? kPieceMain
+           : k0 + kGK <= w                      ? kPieceS1
+           : k0 >= w + nm                       ? kPieceS2
+                                                : kPieceMixed;
+}
+
+// Buffer resources (A, B arrays) of one piece's domain block.
+// (Selections are between values, never between struct members: a select of
+// member lvalues keeps the struct in scratch memory.)
+template <class T>
+__device__ __forceinline__ T pick3(int piece, T main, T s1, T s2) {
+    return piece == kPieceMain ? main : piece == kPieceS1 ? s1 : s2;
+}
+__device__ __forceinline__ void piece_rsrc(const GemmSrc &src, const TileDesc &td, int piece, Rsrc &rA, Rsrc &rB) {
+    const long long dom = pick3<long long>(piece, td.m_dom, td.s1_dom, td.s2_dom);
+    const int size = pick3<int>(piece, td.m_size, td.s1_size, td.s2_size);
+    const int ab = pick3<int>(piece, td.m_b - td.m_a, td.s1_b - td.s1_a, td.s2_b - td.s2_a);
+    rA = block_rsrc(src.A, dom, size);
+    rB = block_rsrc(src.B, dom + ab, size - ab);
+}
+
+template <int DIR, bool HASB>
+__device__ __forceinline__ void gemm_load_u(const GemmSrc &src, const TileDesc &td, const LineAddr &L, bool ok, int k0,
+                                            Rsrc dotr, bool dot, UChunk &u) {
+    const int kb = k0 + stage_kq<DIR>(threadIdx.x);
+    const int piece = chunk_piece(td, L.K, k0);
+    if (piece == kPieceMixed) {
+        // a chunk across a piece boundary: each value from whichever block holds it
+        Rsrc mA, mB, s1A, s1B, s2A, s2B;
+        piece_rsrc(src, td, kPieceMain, mA, mB);
+        piece_rsrc(src, td, kPieceS1, s1A, s1B);
+        piece_rsrc(src, td, kPieceS2, s2A, s2B);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = kb + j;
+            const int om = ok ? L.main_off(k) : kOOB, o1 = ok ? L.s1_off(k) : kOOB, o2 = ok ? L.s2_off(k) : kOOB;
+            u.a[j] = bload(mA, om) + bload(s1A, o1) + bload(s2A, o2);
+            u.b[j] = HASB ? bload(mB, om) + bload(s1B, o1) + bload(s2B, o2) : 0.f;
+        }
+    } else {
+        // one block for the whole chunk
+        Rsrc rA, rB;
+        piece_rsrc(src, td, piece, rA, rB);
+        const bool in = ok && (piece == kPieceMain || (piece == kPieceS1 && L.s1) || (piece == kPieceS2 && L.s2));
+        const int base = pick3<int>(piece, L.m_base, L.s1_base, L.s2_base);
+        const int ps = pick3<int>(piece, L.m_ps, L.s1_ps, L.s2_ps);
+        const int off = in && piece != kPieceNone ? base + kb * ps : kOOB;
+        if (DIR == 0 && piece != kPieceS2) {
+            // X lines inside a row: two 16-byte loads per array
+            const float4 x0 = bload4(rA, off), x1 = bload4(rA, off == kOOB ? kOOB : off + 16);
+            u.a[0] = x0.x; u.a[1] = x0.y; u.a[2] = x0.z; u.a[3] = x0.w;
+            u.a[4] = x1.x; u.a[5] = x1.y; u.a[6] = x1.z; u.a[7] = x1.w;
+            if constexpr (HASB) {
+                const float4 y0 = bload4(rB, off), y1 = bload4(rB, off == kOOB ? kOOB : off + 16);
+                u.b[0] = y0.x; u.b[1] = y0.y; u.b[2] = y0.z; u.b[3] = y0.w;
+                u.b[4] = y1.x; u.b[5] = y1.y; u.b[6] = y1.z; u.b[7] = y1.w;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) u.b[j] = 0.f;
+            }
+        } else {
+            // Y lines (a wavefront reads a row segment per k), or the side-2
+            // chunks of X lines (they may run to the end of the block): 4-byte loads
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int o = off == kOOB || kb + j >= L.K ? kOOB : off + j * ps;
+                u.a[j] = bload(rA, o);
+                u.b[j] = HASB ? bload(rB, o) : 0.f;
+            }
+        }
+    }
+    if (dot) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) u.dk[j] = bload(dotr, (kb + j) * 4);  // zero from K on
+    }
+}
+
+__device__ __forceinline__ void gemm_load_d(const Rsrc &dh, const Rsrc &dl, int n0, int k_pad, int k0, DChunk &d) {
     const int tid = threadIdx.x;
-    float v0[16], v1[16];
-    split_load_chunk(g.st, td, gd.K, pc.k0, v0);
-    split_load_chunk(g.st, td, gd.K, pc.k0 + kCK, v1);
-    if (tid < kGB) emax[tid] = 0;
-    __syncthreads();
+    // vectors i = 0, 1 of the hi part, 2, 3 of the lo part: rows (tid >> 2) + 128 (i & 1), piece tid & 3
 #pragma unroll
-    for (int i = 0; i < 16; ++i)
-        atomicMax(&emax[split_line(td, i)], __float_as_int(fmaxf(fabsf(v0[i]), fabsf(v1[i]))));
-    __syncthreads();
-    if (tid < kGB) {
-        atomicMax(&umax[parity * n_slots + gd.e_off + tid], emax[tid]);
-        if (pc.k0 == 0) umax[(1 - parity) * n_slots + gd.e_off + tid] = 0;
+    for (int i = 0; i < 4; ++i) {
+        const int row = (tid >> 2) + (i & 1) * (kGN / 2), c = tid & 3;
+        const int off = (((n0 + row) * k_pad) + k0 + 8 * c) * 2;
+        d.v[i] = __builtin_amdgcn_raw_buffer_load_b128(i < 2 ? dh : dl, off, 0, 0);
     }
 }
 
-__global__ void __launch_bounds__(kGemmThreads) stage_usplit_kernel(GemmArgs g, const SplitPiece *__restrict__ pieces,
-                                                                    const int *umax, long long n_slots, int parity) {
-    __shared__ __attribute__((aligned(16))) _Float16 th[kGB * kGKP], tl[kGB * kGKP];
-    __shared__ float escale[kGB];
-    const SplitPiece pc = pieces[blockIdx.x];
-    const GemmDesc gd = g.tiles[pc.lb];
-    const TileDesc &td = gd.t;
-    const int tid = threadIdx.x;
-    float v[2][16];
-    split_load_chunk(g.st, td, gd.K, pc.k0, v[0]);
-    split_load_chunk(g.st, td, gd.K, pc.k0 + kCK, v[1]);
-    if (tid < kGB) {
-        const float m = __int_as_float(umax[parity * n_slots + gd.e_off + tid]);
-        const int e = m > 0.f ? min(max(14 - ilogbf(m), -100), 100) : 0;
-        escale[tid] = ldexpf(1.f, e);
-        if (pc.k0 == 0) g.uexp[gd.e_off + tid] = e;
+// Converts and writes one chunk into LDS buffer `buf` (8 k of one line per
+// thread as one 16-byte write per part); accumulates the FP32 dot product of
+// the last output row (dot).
+template <typename E, int DIR, bool HASB>
+__device__ __forceinline__ void gemm_stage(E *ulds, E *dlds, int buf, const UChunk &u, const DChunk &d, float sc,
+                                           bool dot, float &gv) {
+    using V8 = typename MfmaOps<E>::v8;
+    const int tid = threadIdx.x, row = stage_line<DIR>(tid), kq = stage_kq<DIR>(tid);
+    E *U0 = ulds + (buf * 2 + 0) * kGL * kGP, *U1 = ulds + (buf * 2 + 1) * kGL * kGP;
+    V8 hi, lo;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float raw = HASB ? u.a[j] + u.b[j] : u.a[j];
+        const float x = raw * sc;
+        const E h = (E)x;
+        hi[j] = h;
+        lo[j] = (E)(x - (float)h);
+        if (dot) gv = __builtin_fmaf(raw, u.dk[j], gv);
     }
-    __syncthreads();
+    *reinterpret_cast<V8 *>(U0 + row * kGP + kq) = hi;
+    *reinterpret_cast<V8 *>(U1 + row * kGP + kq) = lo;
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int line = split_line(td, i), kk = h * kCK + split_k(td, i);
-            const float xs = v[h][i] * escale[line];
-            const _Float16 hi = (_Float16)xs;
-            th[line * kGKP + kk] = hi;
-            tl[line * kGKP + kk] = (_Float16)(xs - (float)hi);
-        }
-    __syncthreads();
-    // 128 rows x 64 halfs per part: 1024 vectors of 8 halfs, four per thread and part
-    _Float16 *Uh = g.umat + gd.u_hi, *Ul = g.umat + gd.u_lo;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int vi = tid + j * kGemmThreads, row = vi >> 3, col = (vi & 7) * 8;
-        const size_t o = (size_t)row * gd.k_pad + pc.k0 + col;
-        *reinterpret_cast<half8 *>(Uh + o) = *reinterpret_cast<const half8 *>(&th[row * kGKP + col]);
-        *reinterpret_cast<half8 *>(Ul + o) = *reinterpret_cast<const half8 *>(&tl[row * kGKP + col]);
+    for (int i = 0; i < 4; ++i) {
+        const int part = i >> 1, drow = (tid >> 2) + (i & 1) * (kGN / 2), c = tid & 3;
+        *reinterpret_cast<u32x4 *>(dlds + ((buf * 2 + part) * kGN + drow) * kGP + 8 * c) = d.v[i];
     }
 }
 
-// MK: k per LDS chunk (32: 40 KB of LDS, three blocks per CU; 64: 74 KB, two).
-template <int MK>
-__global__ void __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(MK == 32 ? 3 : 2)))
-stage_gemm_kernel(GemmArgs g) {
-    constexpr int MKP = MK + 8;        // LDS row pitch in halfs
-    constexpr int NV = MK / 8 / 2;     // 16-byte vectors per thread and part (128 rows x MK halfs)
-    __shared__ __attribute__((aligned(16))) _Float16 uh[kGB * MKP], ul[kGB * MKP], dh[kGB * MKP], dl[kGB * MKP];
-    __shared__ int uexp[kGB];
-    const StageArgs &a = g.st;
-    const GemmDesc gd = g.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
-    const TileDesc &td = gd.t;
-    const int tid = threadIdx.x, nch = gd.k_pad / MK;
-    if (tid < kGB) uexp[tid] = g.uexp[gd.e_off + tid];
-    // operand chunks: 128 rows x MK halfs per part, NV vectors of 8 halfs per thread and part
-    const _Float16 *Uh = g.umat + gd.u_hi, *Ul = g.umat + gd.u_lo;
-    const _Float16 *Dh = g.dmat + gd.d_hi + (size_t)gd.n0 * gd.k_pad, *Dl = g.dmat + gd.d_lo + (size_t)gd.n0 * gd.k_pad;
-    half8 pu_h[NV], pu_l[NV], pd_h[NV], pd_l[NV];
-    auto load_chunk = [&](int c) {
+// The MFMAs of one chunk (LDS buffer `buf`): wave (lg, og) accumulates lines
+// 64 lg .. +63 x outputs 64 og .. +63 (2 x 2 blocks of 32 x 32).
+template <typename E, int DIR>
+__device__ __forceinline__ void gemm_chunk(const E *ulds, const E *dlds, int buf, int lg, int og, f32x16 (&acc)[2][2]) {
+    using V8 = typename MfmaOps<E>::v8;
+    const int lane = threadIdx.x & 63;
+    const E *U0 = ulds + (buf * 2 + 0) * kGL * kGP, *U1 = ulds + (buf * 2 + 1) * kGL * kGP;
+    const E *D0 = dlds + (buf * 2 + 0) * kGN * kGP, *D1 = dlds + (buf * 2 + 1) * kGN * kGP;
+    // rows of C from R (U for X, D for Y) at r_base, columns from Q at q_base
+    const E *R0 = DIR == 0 ? U0 : D0, *R1 = DIR == 0 ? U1 : D1;
+    const E *Q0 = DIR == 0 ? D0 : U0, *Q1 = DIR == 0 ? D1 : U1;
+    const int r_base = DIR == 0 ? 64 * lg : 64 * og, q_base = DIR == 0 ? 64 * og : 64 * lg;
 #pragma unroll
-        for (int j = 0; j < NV; ++j) {
-            const int vi = tid + j * kGemmThreads, row = vi / (MK / 8), col = (vi % (MK / 8)) * 8;
-            const size_t o = (size_t)row * gd.k_pad + c * MK + col;
-            pu_h[j] = *reinterpret_cast<const half8 *>(Uh + o);
-            pu_l[j] = *reinterpret_cast<const half8 *>(Ul + o);
-            pd_h[j] = *reinterpret_cast<const half8 *>(Dh + o);
-            pd_l[j] = *reinterpret_cast<const half8 *>(Dl + o);
+    for (int s = 0; s < kGK / 16; ++s) {
+        const int kk = s * 16 + (lane >> 5) * 8;
+        V8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int rr = (r_base + 32 * i + (lane & 31)) * kGP + kk, qq = (q_base + 32 * i + (lane & 31)) * kGP + kk;
+            ah[i] = *reinterpret_cast<const V8 *>(R0 + rr);
+            al[i] = *reinterpret_cast<const V8 *>(R1 + rr);
+            bh[i] = *reinterpret_cast<const V8 *>(Q0 + qq);
+            bl[i] = *reinterpret_cast<const V8 *>(Q1 + qq);
         }
-    };
-    load_chunk(0);
-    const int wv = tid >> 6, lane = tid & 63, wr = wv >> 1, wc = wv & 1;
+        // each product over the four independent accumulators in turn (no
+        // back-to-back MFMAs on one accumulator)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = MfmaOps<E>::mfma(ah[i], bh[j], acc[i][j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = MfmaOps<E>::mfma(ah[i], bl[j], acc[i][j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = MfmaOps<E>::mfma(al[i], bh[j], acc[i][j]);
+    }
+}
+
+template <typename E, int DIR, bool HASB>
+__device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt, const float *ftab,
+                                          unsigned char *gemm_lds) {
+    E *ulds = reinterpret_cast<E *>(gemm_lds);
+    E *dlds = ulds + kGemmLdsU;
+    const TileDesc &td = gt.t;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lg = wave >> 2, og = wave & 3;
+    const int e = gemm_scale_exp(a, td.scene, td.vel);
+    const float sc = ldexpf(1.f, e);
+    const GemmSrc r = gemm_src(a, td);
+    const Rsrc dh = bytes_rsrc(a.dmat, gt.d_hi * 2, gt.n_rows * gt.k_pad * 2),
+               dl = bytes_rsrc(a.dmat, gt.d_lo * 2, gt.n_rows * gt.k_pad * 2);
+    const bool dot = gt.dot_off >= 0;
+    const Rsrc dotr = bytes_rsrc(ftab, dot ? (long long)gt.dot_off * 4 : 0, dot ? gt.K * 4 : 0);
+    const int nch = gt.k_pad / kGK;
+    const bool line_ok = stage_line<DIR>(tid) < td.count;
+    const LineAddr L(td, td.line0 + stage_line<DIR>(tid), gt.K);
+    // waves whose lines or outputs lie outside the tile skip their MFMAs
+    const bool active = 64 * lg < td.count && 64 * og < gt.n_g - gt.n0;
     f32x16 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
-    // rows of C come from R (U for X lines, D for Y lines), columns from Q
-    const _Float16 *Rh = td.dir == 0 ? uh : dh, *Rl = td.dir == 0 ? ul : dl;
-    const _Float16 *Qh = td.dir == 0 ? dh : uh, *Ql = td.dir == 0 ? dl : ul;
+            for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.f;
+    float gv = 0.f;
+    // One register set: the loads of chunk c+1 are issued right after the
+    // barrier that makes chunk c visible, and are in flight while the MFMAs of
+    // chunk c run; then chunk c+1 is converted into the other LDS buffer.
+    UChunk u;
+    DChunk d;
+    gemm_load_u<DIR, HASB>(r, td, L, line_ok, 0, dotr, dot, u);
+    gemm_load_d(dh, dl, gt.n0, gt.k_pad, 0, d);
+    gemm_stage<E, DIR, HASB>(ulds, dlds, 0, u, d, sc, dot, gv);
+    __syncthreads();
     for (int c = 0; c < nch; ++c) {
-        __syncthreads();  // the previous chunk's MFMA reads are done
-#pragma unroll
-        for (int j = 0; j < NV; ++j) {
-            const int vi = tid + j * kGemmThreads, row = vi / (MK / 8), col = (vi % (MK / 8)) * 8;
-            *reinterpret_cast<half8 *>(&uh[row * MKP + col]) = pu_h[j];
-            *reinterpret_cast<half8 *>(&ul[row * MKP + col]) = pu_l[j];
-            *reinterpret_cast<half8 *>(&dh[row * MKP + col]) = pd_h[j];
-            *reinterpret_cast<half8 *>(&dl[row * MKP + col]) = pd_l[j];
+        const int buf = c & 1;
+        const bool next = c + 1 < nch;
+        if (next) {
+            gemm_load_u<DIR, HASB>(r, td, L, line_ok, (c + 1) * kGK, dotr, dot, u);
+            gemm_load_d(dh, dl, gt.n0, gt.k_pad, (c + 1) * kGK, d);
         }
+        if (active) gemm_chunk<E, DIR>(ulds, dlds, buf, lg, og, acc);
+        if (next) gemm_stage<E, DIR, HASB>(ulds, dlds, buf ^ 1, u, d, sc, dot, gv);
         __syncthreads();
-        if (c + 1 < nch) load_chunk(c + 1);
-#pragma unroll
-        for (int s = 0; s < MK / 16; ++s) {
-            const int kk = s * 16 + (lane >> 5) * 8;
-            half8 ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int row = wr * 64 + i * 32 + (lane & 31);
-                ah[i] = *reinterpret_cast<const half8 *>(&Rh[row * MKP + kk]);
-                al[i] = *reinterpret_cast<const half8 *>(&Rl[row * MKP + kk]);
-                const int col = wc * 64 + i * 32 + (lane & 31);
-                bh[i] = *reinterpret_cast<const half8 *>(&Qh[col * MKP + kk]);
-                bl[i] = *reinterpret_cast<const half8 *>(&Ql[col * MKP + kk]);
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                }
-        }
     }
-    // epilogue: L = C / (2^e_line 2^d_exp); out = t_n - s*coef*L (Solver::update_field_values)
-    const float sc = a.s * td.coef;
+    // epilogue: L = C / (2^e 2^d_exp); out = t_n - s*coef*L (Solver::update_field_values)
+    const float sc_upd = a.s * td.coef, inv = ldexpf(1.f, -(e + gt.d_exp));
     const bool use_pml = a.last && td.is_pml;
     const Rsrc rt = block_rsrc(a.tn, td.m_dom, td.m_size);
     const Rsrc rp = block_rsrc(a.pml, td.m_dom, td.m_size);
     const Rsrc ro = block_rsrc(a.out, td.m_dom, td.m_size);
-    // eight values at a time (fewer live registers than a whole 32x32 tile)
+    float mx = 0.f;
+    if (active) {
+        // value q of block (i, j): row 32 i + 8 (q >> 2) + 4 (lane >> 5) + (q & 3), column 32 j + (lane & 31)
+        // of the wave's 64 x 64; all t_n loads are issued before the first is used
+        const int lrow0 = DIR == 0 ? 64 * lg + 4 * (lane >> 5) : 64 * lg + (lane & 31);
+        const int nrow0 = gt.n0 + (DIR == 0 ? 64 * og + (lane & 31) : 64 * og + 4 * (lane >> 5));
+        auto offset = [&](int i, int j, int q, bool &ok) {
+            const int rowd = 32 * i + 8 * (q >> 2) + (q & 3), cold = 32 * j;
+            const int line = lrow0 + (DIR == 0 ? rowd : cold), n = nrow0 + (DIR == 0 ? cold : rowd);
+            ok = line < td.count && n < gt.n_g;
+            return ok ? (td.o_f + (td.line0 + line) * td.o_ls + n * td.o_ps) * 4 : kOOB;
+        };
+        float tnv[2][2][16];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int hh = 0; hh < 2; ++hh) {
-                int off[8];
-                float tnv[8], pm[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int r = hh * 8 + q;
-                    const int row = wr * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                    const int col = wc * 64 + j * 32 + (lane & 31);
-                    const int line = td.dir == 0 ? row : col, n = gd.n0 + (td.dir == 0 ? col : row);
-                    const bool ok = line < td.count && n < td.n_out;
-                    off[q] = ok ? (td.o_f + (td.line0 + line) * td.o_ls + n * td.o_ps) * 4 : kOOB;
-                    tnv[q] = bload(rt, off[q]);
-                    pm[q] = use_pml ? bload(rp, off[q]) : 1.f;
+                for (int q = 0; q < 16; ++q) {
+                    bool ok;
+                    tnv[i][j][q] = bload(rt, offset(i, j, q, ok));
                 }
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int r = hh * 8 + q;
-                    const int row = wr * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                    const int col = wc * 64 + j * 32 + (lane & 31);
-                    const int line = td.dir == 0 ? row : col;
-                    const float L = ldexpf(acc[i][j][r], -(uexp[line] + gd.d_exp));
-                    const float nv = __builtin_fmaf(-sc, L, tnv[q]);
-                    bstore(ro, off[q], nv);
-                    if (a.last) bstore(rt, off[q], nv * pm[q]);
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    bool ok;
+                    const int off = offset(i, j, q, ok);
+                    const float nv = __builtin_fmaf(-sc_upd, acc[i][j][q] * inv, tnv[i][j][q]);
+                    bstore(ro, off, nv);
+                    if (a.last) bstore(rt, off, nv * (use_pml ? bload(rp, off) : 1.f));
+                    if (ok) mx = fmaxf(mx, fabsf(nv));
                 }
+    }
+    if (dot) {
+        // output n_out - 1 of every line: the FP32 dot products (no scaling)
+        const int n = gt.n_g;
+        float Ldot = 0.f;
+        int line = -1;
+        if constexpr (DIR == 0) {
+            // the 4 threads of a line are adjacent lanes
+            float g = gv;
+            g += __shfl_xor(g, 1);
+            g += __shfl_xor(g, 2);
+            if ((tid & 3) == 0) {
+                line = tid >> 2;
+                Ldot = g;
             }
+        } else {
+            // partial sums of the 4 k-groups through LDS (the ring is no longer read)
+            float *red = reinterpret_cast<float *>(gemm_lds);
+            red[(tid >> 7) * kGL + (tid & (kGL - 1))] = gv;
+            __syncthreads();
+            if (tid < kGL) {
+                line = tid;
+#pragma unroll
+                for (int k = 0; k < kGemmThreads / kGL; ++k) Ldot += red[k * kGL + tid];
+            }
+        }
+        if (line >= 0 && line < td.count) {
+            const int off = (td.o_f + (td.line0 + line) * td.o_ls + n * td.o_ps) * 4;
+            const float tnv = bload(rt, off), pm = use_pml ? bload(rp, off) : 1.f;
+            const float nv = __builtin_fmaf(-sc_upd, Ldot, tnv);
+            bstore(ro, off, nv);
+            if (a.last) bstore(rt, off, nv * pm);
+            mx = fmaxf(mx, fabsf(nv));
+        }
+    }
+    track_max(a, td.scene, td.vel ? 0 : 1, mx);
+}
+
+template <typename E>
+__global__ void __launch_bounds__(kGemmThreads) stage_gemm_kernel(StageArgs a, const GemmTile *__restrict__ tiles,
+                                                                  const float *__restrict__ ftab) {
+    // clears the slot the next stage writes (nothing reads or writes it in this stage)
+    if (blockIdx.x == 0 && a.umax)
+        for (int i = threadIdx.x; i < 2 * a.n_scenes; i += blockDim.x) a.umax[(size_t)a.slot_z * a.n_scenes * 2 + i] = 0u;
+    extern __shared__ __attribute__((aligned(16))) unsigned char gemm_lds[];
+    // XCD-contiguous runs of tiles always (the tiles of one matrix and of one
+    // line block share an XCD's L2), whatever order the FFT-form tiles use
+    const GemmTile gt = tiles[tile_index(blockIdx.x, gridDim.x, 1)];
+    const bool hasB = !gt.t.vel && a.p0b != nullptr;  // p0 = px + py
+    if (gt.t.dir == 0) {
+        if (hasB) gemm_tile<E, 0, true>(a, gt, ftab, gemm_lds);
+        else gemm_tile<E, 0, false>(a, gt, ftab, gemm_lds);
+    } else {
+        if (hasB) gemm_tile<E, 1, true>(a, gt, ftab, gemm_lds);
+        else gemm_tile<E, 1, false>(a, gt, ftab, gemm_lds);
+    }
+}
+
+// Largest |p0| and |vx|, |vy| of every scene in the state a first stage reads
+// after the host wrote fields (it has no previous stage): into slot `slot`.
+struct MaxRange {
+    long long off;  // element offset from the arena base
+    long long n;
+    int idx;        // scene * 2 + class
+};
+
+__global__ void state_max_kernel(const float *__restrict__ arena, unsigned *__restrict__ umax,
+                                 const MaxRange *__restrict__ ranges, int nr, int n_scenes, int slot) {
+    for (int k = blockIdx.y; k < nr; k += gridDim.y) {
+        const MaxRange r = ranges[k];
+        float m = 0.f;
+        for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < r.n; e += (long long)gridDim.x * blockDim.x)
+            m = fmaxf(m, fabsf(arena[r.off + e]));
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(&umax[(size_t)slot * n_scenes * 2 + r.idx], __float_as_uint(m));
+    }
 }
 
 // ------------------------------------------------------------- host side
@@ -980,11 +1203,10 @@ struct pstd_ctx {
     std::vector<TileDesc> fft_segs, tiles_extra, tiles_all;
     int small_nt = 256;
     int small_fmax = 0;  // largest F of the F <= 2048 class (128 threads need F <= 1024)
-    std::vector<GemmDesc> tiles_gemm, tiles_split;
-    std::vector<SplitPiece> pieces;
-    long long u_halfs = 0, u_exps = 0;  // split-input scratch of the GEMM form
+    std::vector<GemmTile> tiles_gemm;
+    int n_scenes = 1;
+    std::vector<MaxRange> max_ranges;  // state read by a first stage after host writes (GEMM form)
     int form = PSTD_FORM_FFT;
-    int gemm_mk = 64;        // k per LDS chunk of the GEMM kernel (autotuned: 32 or 64)
     double gemm_flops = 0;   // algorithmic MFMA flops of one stage (3 split products)
     long long gemm_cells = 0;
     size_t arena_floats = 0;
@@ -997,12 +1219,10 @@ struct pstd_ctx {
     float2 *ctab = nullptr;
     float *ftab = nullptr;
     TileDesc *d_tiles = nullptr;  // small | 4096 | 8192
-    GemmDesc *d_gemm = nullptr, *d_split = nullptr;
-    _Float16 *dmat = nullptr;     // derivative matrices of the GEMM form (hi, lo parts)
-    _Float16 *umat = nullptr;     // split input lines of the GEMM form (rewritten every stage)
-    int *uexp = nullptr;
-    int *umax = nullptr;          // two sets of per-line maxima (float bits), alternating by stage
-    SplitPiece *d_pieces = nullptr;
+    GemmTile *d_gemm = nullptr;
+    void *dmat = nullptr;         // derivative matrices of the GEMM form (hi, lo parts; 2-byte elements)
+    unsigned *umax = nullptr;     // per-scene maxima of the written values, 3 slots x scenes x 2 classes
+    MaxRange *d_max_ranges = nullptr;
     ReceiverCell *d_cells = nullptr;
     P0Range *d_ranges = nullptr;
     int n_ranges = 0;
@@ -1013,6 +1233,10 @@ struct pstd_ctx {
     int xcd_remap = 1;
     int lean = 0;  // stage kernel flavour (kEarly, kLate, kMid)
     float tuned_stage_ms = 0;
+    // per-kernel-class timing (pstd_ctx_profile_kernels): event pairs around
+    // every launch of a class while `prof` is set
+    bool prof = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_events;
     // hipGraph of `graph_steps` whole time steps (0: launches are issued one by one)
     int graph_steps = 0;
     hipGraph_t graph = nullptr;
@@ -1037,17 +1261,13 @@ struct pstd_ctx {
         (void)hipFree(ftab);
         (void)hipFree(d_tiles);
         (void)hipFree(d_gemm);
-        (void)hipFree(d_split);
         (void)hipFree(dmat);
-        (void)hipFree(umat);
-        (void)hipFree(uexp);
         (void)hipFree(umax);
-        (void)hipFree(d_pieces);
+        (void)hipFree(d_max_ranges);
+        d_gemm = nullptr;
+        dmat = nullptr;
         umax = nullptr;
-        d_pieces = nullptr;
-        d_gemm = d_split = nullptr;
-        dmat = umat = nullptr;
-        uexp = nullptr;
+        d_max_ranges = nullptr;
         (void)hipFree(d_cells);
         (void)hipFree(d_ranges);
         (void)hipFree(d_samples);
@@ -1082,6 +1302,27 @@ void build_window(int w, int patch_error, std::vector<float> &out) {
     out.assign(win.begin(), win.end());
 }
 
+// Kernel classes of pstd_ctx_profile_kernels.
+constexpr int kClassSmall = 0, kClass4096 = 1, kClass8192 = 2, kClassGemm = 3, kClasses = 4;
+
+// Brackets the launches issued by `launch` with an event pair when profiling.
+template <class Fn>
+void timed_launch(pstd_ctx *ctx, int cls, Fn launch) {
+    if (!ctx->prof) {
+        launch();
+        return;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        launch();
+        return;
+    }
+    (void)hipEventRecord(e0, ctx->stream);
+    launch();
+    (void)hipEventRecord(e1, ctx->stream);
+    ctx->prof_events.push_back({cls, {e0, e1}});
+}
+
 // One RK stage.  sample_slot >= 0 (first stage only): the launch also records
 // the receivers of the previous step into that sample slot.
 int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
@@ -1113,9 +1354,25 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
     a.s = ctx->dt * ctx->alpha[k];
     a.last = k == 5;
     a.remap = ctx->xcd_remap;
+    const int ng = (int)ctx->tiles_gemm.size();
+    if (ng) {
+        // stage k writes slot k%3, reads (k+2)%3 and clears (k+1)%3 (6 stages per
+        // step: the same slots in every step, so a captured graph replays them)
+        a.umax = ctx->umax;
+        a.slot_w = k % 3;
+        a.slot_r = (k + 2) % 3;
+        a.slot_z = (k + 1) % 3;
+        a.n_scenes = ctx->n_scenes;
+        a.dmat = ctx->dmat;
+        if (k == 0 && ctx->p0_explicit && !ctx->max_ranges.empty()) {
+            const int nr = (int)ctx->max_ranges.size();
+            hipLaunchKernelGGL(state_max_kernel, dim3(16, std::min(nr, 4096)), dim3(256), 0, ctx->stream, ctx->arena,
+                               ctx->umax, ctx->d_max_ranges, nr, ctx->n_scenes, a.slot_r);
+        }
+    }
     const int ns = (int)ctx->tiles_small.size(), n4 = (int)ctx->tiles_4096.size(), n8 = (int)ctx->tiles_8192.size();
     const int fl = ctx->lean;
-    if (ns) {
+    if (ns) timed_launch(ctx, kClassSmall, [&] {
         a.tiles = ctx->d_tiles;
         if (ctx->small_nt == 512) {
             if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<512, kLate>), dim3(ns), dim3(512), 0, ctx->stream, a);
@@ -1130,39 +1387,29 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
             else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<256, kMid>), dim3(ns), dim3(256), 0, ctx->stream, a);
             else hipLaunchKernelGGL((stage_small_kernel<256, kEarly>), dim3(ns), dim3(256), 0, ctx->stream, a);
         }
-    }
-    if (n4) {
+    });
+    if (n4) timed_launch(ctx, kClass4096, [&] {
         a.tiles = ctx->d_tiles + ns;
         size_t lds = 2 * sizeof(float2) * LdsMap<4096, 1, 0>::SIZE;
         if (fl == kLate) hipLaunchKernelGGL((stage_big_kernel<4096, kLate>), dim3(n4), dim3(512), lds, ctx->stream, a);
         else if (fl == kMid) hipLaunchKernelGGL((stage_big_kernel<4096, kMid>), dim3(n4), dim3(512), lds, ctx->stream, a);
         else hipLaunchKernelGGL((stage_big_kernel<4096, kEarly>), dim3(n4), dim3(512), lds, ctx->stream, a);
-    }
-    if (n8) {
+    });
+    if (n8) timed_launch(ctx, kClass8192, [&] {
         a.tiles = ctx->d_tiles + ns + n4;
         size_t lds = 2 * sizeof(float2) * LdsMap<8192, 1, 0>::SIZE;
         if (fl == kLate) hipLaunchKernelGGL((stage_big_kernel<8192, kLate>), dim3(n8), dim3(1024), lds, ctx->stream, a);
         else if (fl == kMid) hipLaunchKernelGGL((stage_big_kernel<8192, kMid>), dim3(n8), dim3(1024), lds, ctx->stream, a);
         else hipLaunchKernelGGL((stage_big_kernel<8192, kEarly>), dim3(n8), dim3(1024), lds, ctx->stream, a);
-    }
-    const int ng = (int)ctx->tiles_gemm.size(), nsp = (int)ctx->tiles_split.size();
-    if (ng) {
-        GemmArgs ga;
-        ga.st = a;
-        ga.dmat = ctx->dmat;
-        ga.umat = ctx->umat;
-        ga.uexp = ctx->uexp;
-        ga.tiles = ctx->d_split;
-        const int np = (int)ctx->pieces.size(), parity = k & 1;
-        (void)nsp;
-        hipLaunchKernelGGL(stage_umax_kernel, dim3(np), dim3(kGemmThreads), 0, ctx->stream, ga, ctx->d_pieces,
-                           ctx->umax, ctx->u_exps, parity);
-        hipLaunchKernelGGL(stage_usplit_kernel, dim3(np), dim3(kGemmThreads), 0, ctx->stream, ga, ctx->d_pieces,
-                           ctx->umax, ctx->u_exps, parity);
-        ga.tiles = ctx->d_gemm;
-        if (ctx->gemm_mk == 32) hipLaunchKernelGGL(stage_gemm_kernel<32>, dim3(ng), dim3(kGemmThreads), 0, ctx->stream, ga);
-        else hipLaunchKernelGGL(stage_gemm_kernel<64>, dim3(ng), dim3(kGemmThreads), 0, ctx->stream, ga);
-    }
+    });
+    if (ng) timed_launch(ctx, kClassGemm, [&] {
+        if (ctx->form == PSTD_FORM_GEMM_BF16)
+            hipLaunchKernelGGL(stage_gemm_kernel<__bf16>, dim3(ng), dim3(kGemmThreads), kGemmLdsBytes, ctx->stream, a,
+                               ctx->d_gemm, ctx->ftab);
+        else
+            hipLaunchKernelGGL(stage_gemm_kernel<_Float16>, dim3(ng), dim3(kGemmThreads), kGemmLdsBytes, ctx->stream, a,
+                               ctx->d_gemm, ctx->ftab);
+    });
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx->err = std::string("stage launch: ") + hipGetErrorString(e);
@@ -1231,6 +1478,9 @@ int pstd_ctx_create(int device, pstd_ctx **out) {
     for (int f = 0; f < 3; ++f)
         lds_ok = lds_ok && hipFuncSetAttribute(big8[f], attr, lds8) == hipSuccess &&
                  hipFuncSetAttribute(big4[f], attr, lds4) == hipSuccess;
+    lds_ok = lds_ok &&
+             hipFuncSetAttribute((const void *)stage_gemm_kernel<_Float16>, attr, kGemmLdsBytes) == hipSuccess &&
+             hipFuncSetAttribute((const void *)stage_gemm_kernel<__bf16>, attr, kGemmLdsBytes) == hipSuccess;
     if (!lds_ok) {
         ctx->err = "cannot reserve LDS for the long-line kernels";
     }
@@ -1365,13 +1615,6 @@ static int autotune(pstd_ctx *ctx) {
             rc = upload_tiles(ctx);
         }
     }
-    if (rc == PSTD_OK && !ctx->tiles_gemm.empty()) {  // GEMM chunk depth, with the chosen flavour and order
-        ctx->gemm_mk = 32;
-        float ms = 0;
-        rc = time_steps(ms);
-        if (rc == PSTD_OK && ms < best) best = ms;
-        else ctx->gemm_mk = 64;
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) {
@@ -1380,21 +1623,63 @@ static int autotune(pstd_ctx *ctx) {
     }
     ctx->tuned_stage_ms = best / (6.0 * n);
     HIP_TRY(hipMemsetAsync(ctx->arena, 0, 4 * (size_t)ctx->arena_floats * sizeof(float), ctx->stream));
+    if (ctx->umax) HIP_TRY(hipMemsetAsync(ctx->umax, 0, 3 * 2 * (size_t)ctx->n_scenes * sizeof(unsigned), ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->steps_done = 0;
     ctx->p0_explicit = true;
     return PSTD_OK;
 }
 
+// 16-bit parts of the GEMM form, from double, rounded to nearest even.
+static uint16_t to_f16_bits(double v) {
+    const _Float16 h = (_Float16)v;
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return b;
+}
+static double from_f16_bits(uint16_t b) {
+    _Float16 h;
+    std::memcpy(&h, &b, 2);
+    return (double)h;
+}
+static uint16_t to_bf16_bits(double v) {
+    if (v == 0.0) return 0;
+    int e = 0;
+    const double m = std::frexp(v, &e);                        // |m| in [0.5, 1)
+    const double q = std::nearbyint(std::ldexp(m, 8));         // 8 significant bits, ties to even
+    const float f = (float)std::ldexp(q, e - 8);               // exact in float
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return (uint16_t)(u >> 16);
+}
+static double from_bf16_bits(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+// Whether a segment runs in the GEMM form: its lines and outputs fill the
+// 128-line x 256-output tiles at least to 3/4 (thin PML strips keep the FFT form).
+static bool gemm_fills(int lines, int n_g) {
+    const double cap = (double)((lines + kGL - 1) / kGL * kGL) * (double)((n_g + kGN - 1) / kGN * kGN);
+    return 4.0 * lines * (double)n_g >= 3.0 * cap;
+}
+
 // GEMM form of one segment: its derivative matrix (built once per distinct
-// segment shape and stencil, split into FP16 hi + lo parts of D * 2^d_exp) and
-// its 128-line x 128-output blocks.
+// segment shape and stencil, split into 16-bit hi + lo parts of D * 2^d_exp;
+// for pressure lines the last row kept apart in FP32 for the dot product) and
+// its 128-line x 256-output tiles.
 static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_plan_segment &s, const TileDesc &g,
-                          std::vector<_Float16> &dmat, std::map<std::string, std::pair<long long, int>> &dcache) {
+                          std::vector<uint16_t> &dmat, std::vector<float> &ftab,
+                          std::map<std::string, std::tuple<long long, int, int>> &dcache) {
     pstdamd::SegmentShape sh{g.F, s.table_len, s.field, (double)plan->dx, s.wlen, g.n_main, (double)plan->patch_error,
                              (double)s.c_side1, (double)s.c_main1, (double)s.c_side2, (double)s.c_main2};
     const int K = sh.K(), n_out = sh.n_out();
-    const int k_pad = (K + kGK - 1) / kGK * kGK, n_pad = (n_out + kGB - 1) / kGB * kGB;  // kGK is a multiple of kCK
+    const bool dot_row = s.field == 0;  // n + 1 outputs: the last one as an FP32 dot product
+    const int n_g = n_out - (dot_row ? 1 : 0);
+    const int k_pad = (K + kGK - 1) / kGK * kGK, n_rows = (n_g + 31) / 32 * 32;
+    const bool bf16 = ctx->form == PSTD_FORM_GEMM_BF16;
     char key[256];
     std::snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%a/%a/%a", sh.F, sh.table_len, sh.field, sh.w, sh.n_main,
                   sh.c_s1, sh.c_m1, sh.c_s2, sh.c_m2);
@@ -1408,49 +1693,57 @@ static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_
             return PSTD_ERR_INVALID;
         }
         double dm = 0;
-        for (double v : D) dm = std::max(dm, std::fabs(v));
+        for (int i = 0; i < n_g; ++i)
+            for (int k = 0; k < K; ++k) dm = std::max(dm, std::fabs(D[(size_t)i * K + k]));
         const int d_exp = dm > 0 ? 14 - std::ilogb(dm) : 0;
-        const long long hi = (long long)dmat.size(), per = (long long)n_pad * k_pad;
-        dmat.resize(dmat.size() + 2 * per, (_Float16)0.f);
-        for (int i = 0; i < n_out; ++i)
+        const long long hi = (long long)dmat.size(), per = (long long)n_rows * k_pad;
+        dmat.resize(dmat.size() + 2 * per, 0);
+        for (int i = 0; i < n_g; ++i)
             for (int k = 0; k < K; ++k) {
                 const double v = std::ldexp(D[(size_t)i * K + k], d_exp);
-                const _Float16 h = (_Float16)v;
+                const uint16_t h = bf16 ? to_bf16_bits(v) : to_f16_bits(v);
+                const double hv = bf16 ? from_bf16_bits(h) : from_f16_bits(h);
                 dmat[hi + (size_t)i * k_pad + k] = h;
-                dmat[hi + per + (size_t)i * k_pad + k] = (_Float16)(v - (double)h);
+                dmat[hi + per + (size_t)i * k_pad + k] = bf16 ? to_bf16_bits(v - hv) : to_f16_bits(v - hv);
             }
-        it = dcache.emplace(key, std::make_pair(hi, d_exp)).first;
+        int dot_off = -1;
+        if (dot_row) {
+            dot_off = (int)ftab.size();
+            for (int k = 0; k < K; ++k) ftab.push_back((float)D[(size_t)(n_out - 1) * K + k]);
+        }
+        it = dcache.emplace(key, std::make_tuple(hi, d_exp, dot_off)).first;
     }
-    const long long per = (long long)n_pad * k_pad;
+    const long long d_hi = std::get<0>(it->second), per = (long long)n_rows * k_pad;
+    if ((long long)n_rows * k_pad * 2 > 0x7fffffffLL) {
+        ctx->err = "GEMM form: derivative matrix above 2 GiB";
+        return PSTD_ERR_UNSUPPORTED;
+    }
     ctx->gemm_flops += 3.0 * 2.0 * K * (double)n_out * (s.line_end - s.line_begin);
     ctx->gemm_cells += (long long)n_out * (s.line_end - s.line_begin);
-    for (int l = s.line_begin; l < s.line_end; l += kGB) {
-        GemmDesc gd{};
-        gd.t = g;
-        gd.t.kind = 3;
-        gd.t.line0 = l;
-        gd.t.count = std::min(kGB, s.line_end - l);
-        gd.d_hi = it->second.first;
-        gd.d_lo = it->second.first + per;
-        gd.u_hi = ctx->u_halfs;
-        gd.u_lo = ctx->u_halfs + (long long)kGB * k_pad;
-        gd.e_off = ctx->u_exps;
-        ctx->u_halfs += 2LL * kGB * k_pad;
-        ctx->u_exps += kGB;
-        gd.k_pad = k_pad;
-        gd.d_exp = it->second.second;
-        gd.K = K;
-        for (int k0 = 0; k0 < k_pad; k0 += kGK) ctx->pieces.push_back({(int)ctx->tiles_split.size(), k0});
-        ctx->tiles_split.push_back(gd);
-        for (int n0 = 0; n0 < n_pad; n0 += kGB) {
-            gd.n0 = n0;
-            ctx->tiles_gemm.push_back(gd);
+    for (int l = s.line_begin; l < s.line_end; l += kGL) {
+        GemmTile gt{};
+        gt.t = g;
+        gt.t.kind = 3;
+        gt.t.line0 = l;
+        gt.t.count = std::min(kGL, s.line_end - l);
+        gt.d_hi = d_hi;
+        gt.d_lo = d_hi + per;
+        gt.n_rows = n_rows;
+        gt.k_pad = k_pad;
+        gt.n_g = n_g;
+        gt.d_exp = std::get<1>(it->second);
+        gt.K = K;
+        for (int n0 = 0; n0 < n_g; n0 += kGN) {
+            gt.n0 = n0;
+            gt.dot_off = n0 + kGN >= n_g ? std::get<2>(it->second) : -1;  // the last output tile
+            ctx->tiles_gemm.push_back(gt);
         }
     }
     return PSTD_OK;
 }
 
-static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
+// dom_scene: scene of every domain (a batch of scenes), or nullptr (one scene).
+static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan, const std::vector<int> *dom_scene = nullptr) {
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->release();
@@ -1463,9 +1756,11 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     ctx->tiles_4096.clear();
     ctx->tiles_8192.clear();
     ctx->tiles_gemm.clear();
-    ctx->tiles_split.clear();
-    ctx->pieces.clear();
-    ctx->u_halfs = ctx->u_exps = 0;
+    ctx->max_ranges.clear();
+    ctx->n_scenes = 1;
+    if (dom_scene)
+        for (int sc : *dom_scene) ctx->n_scenes = std::max(ctx->n_scenes, sc + 1);
+    auto scene_of = [&](int d) { return dom_scene ? (*dom_scene)[d] : 0; };
     ctx->gemm_flops = 0;
     ctx->gemm_cells = 0;
     ctx->plan_domains.assign(plan->domains, plan->domains + plan->n_domains);
@@ -1533,8 +1828,8 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         }
     };
     std::vector<std::vector<char>> covered(plan->n_domains, std::vector<char>(4, 0));
-    std::vector<_Float16> dmat;
-    std::map<std::string, std::pair<long long, int>> dcache;  // matrix key -> (hi offset, d_exp)
+    std::vector<uint16_t> dmat;
+    std::map<std::string, std::tuple<long long, int, int>> dcache;  // matrix key -> (hi offset, d_exp, dot row)
     for (int si = 0; si < plan->n_segments; ++si) {
         const pstd_plan_segment &s = plan->segments[si];
         if (s.domain < 0 || s.domain >= plan->n_domains || s.side1 >= plan->n_domains || s.side2 >= plan->n_domains ||
@@ -1649,12 +1944,13 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
             ftab.insert(ftab.end(), w.begin(), w.end());
         }
         g.win_off = wt->second;
-        // GEMM form: segments whose 128 x 128 blocks are at least 3/4 filled (lines
-        // and outputs) become GEMM blocks; thin ones (PML strips) stay FFT tiles
+        g.scene = scene_of(s.domain);
+        // GEMM form: segments that fill the 128 x 256 tiles at least to 3/4
+        // become GEMM tiles; thin ones (PML strips) stay FFT tiles
         const int nlines_seg = s.line_end - s.line_begin;
-        auto filled = [](int n) { return 4 * n >= 3 * ((n + kGB - 1) / kGB * kGB); };
-        if (ctx->form == PSTD_FORM_GEMM_F16 && filled(nlines_seg) && filled(g.n_out)) {
-            int rc = add_gemm_tiles(ctx, plan, s, g, dmat, dcache);
+        if ((ctx->form == PSTD_FORM_GEMM_F16 || ctx->form == PSTD_FORM_GEMM_BF16) &&
+            gemm_fills(nlines_seg, s.field == 0 ? g.n_out - 1 : g.n_out)) {
+            int rc = add_gemm_tiles(ctx, plan, s, g, dmat, ftab, dcache);
             if (rc) return rc;
             continue;
         }
@@ -1680,6 +1976,8 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
                 t.m_size = d.size;
                 t.o_f = offs[f];
                 t.is_pml = d.is_pml;
+                t.scene = scene_of(i);
+                t.vel = f < 2 ? 1 : 0;  // maxima class: px/py as written by velocity lines
                 ctx->tiles_extra.push_back(t);
             }
         }
@@ -1725,6 +2023,16 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     std::vector<P0Range> ranges;
     for (const DomainDev &d : ctx->doms) ranges.push_back({d.base + d.f_px, d.base + d.f_py, (long long)d.nx * d.ny});
     ctx->n_ranges = (int)ranges.size();
+    if (!ctx->tiles_gemm.empty()) {
+        // what a first stage after host writes reads: p0 (P0I) and vx, vy (TN)
+        for (int i = 0; i < plan->n_domains; ++i) {
+            const DomainDev &d = ctx->doms[i];
+            const int sc = scene_of(i);
+            ctx->max_ranges.push_back({3LL * (long long)ctx->arena_floats + d.base + d.f_px, (long long)d.nx * d.ny, 2 * sc});
+            ctx->max_ranges.push_back({d.base + d.f_vx, (long long)d.ny * (d.nx + 1), 2 * sc + 1});
+            ctx->max_ranges.push_back({d.base + d.f_vy, (long long)(d.ny + 1) * d.nx, 2 * sc + 1});
+        }
+    }
     // allocate + upload
     const size_t af = ctx->arena_floats;
     HIP_TRY(hipMalloc(&ctx->arena, 4 * af * sizeof(float)));
@@ -1740,21 +2048,19 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         if (rc) return rc;
     }
     if (!ctx->tiles_gemm.empty()) {
-        HIP_TRY(hipMalloc(&ctx->d_gemm, ctx->tiles_gemm.size() * sizeof(GemmDesc)));
-        HIP_TRY(hipMemcpyAsync(ctx->d_gemm, ctx->tiles_gemm.data(), ctx->tiles_gemm.size() * sizeof(GemmDesc),
+        // segment order (a scene's segments together, a line block's output
+        // tiles adjacent): measured faster than grouping the tiles by matrix
+        HIP_TRY(hipMalloc(&ctx->d_gemm, ctx->tiles_gemm.size() * sizeof(GemmTile)));
+        HIP_TRY(hipMemcpyAsync(ctx->d_gemm, ctx->tiles_gemm.data(), ctx->tiles_gemm.size() * sizeof(GemmTile),
                                hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(hipMalloc(&ctx->d_split, ctx->tiles_split.size() * sizeof(GemmDesc)));
-        HIP_TRY(hipMemcpyAsync(ctx->d_split, ctx->tiles_split.data(), ctx->tiles_split.size() * sizeof(GemmDesc),
-                               hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(hipMalloc(&ctx->dmat, dmat.size() * sizeof(_Float16)));
-        HIP_TRY(hipMemcpyAsync(ctx->dmat, dmat.data(), dmat.size() * sizeof(_Float16), hipMemcpyHostToDevice,
+        HIP_TRY(hipMalloc(&ctx->dmat, std::max<size_t>(1, dmat.size()) * sizeof(uint16_t)));
+        HIP_TRY(hipMemcpyAsync(ctx->dmat, dmat.data(), dmat.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
                                ctx->stream));
-        HIP_TRY(hipMalloc(&ctx->umat, (size_t)ctx->u_halfs * sizeof(_Float16)));
-        HIP_TRY(hipMalloc(&ctx->uexp, (size_t)ctx->u_exps * sizeof(int)));
-        HIP_TRY(hipMalloc(&ctx->umax, 2 * (size_t)ctx->u_exps * sizeof(int)));
-        HIP_TRY(hipMemsetAsync(ctx->umax, 0, 2 * (size_t)ctx->u_exps * sizeof(int), ctx->stream));
-        HIP_TRY(hipMalloc(&ctx->d_pieces, ctx->pieces.size() * sizeof(SplitPiece)));
-        HIP_TRY(hipMemcpyAsync(ctx->d_pieces, ctx->pieces.data(), ctx->pieces.size() * sizeof(SplitPiece),
+        const size_t nmax = 3 * 2 * (size_t)ctx->n_scenes;
+        HIP_TRY(hipMalloc(&ctx->umax, nmax * sizeof(unsigned)));
+        HIP_TRY(hipMemsetAsync(ctx->umax, 0, nmax * sizeof(unsigned), ctx->stream));
+        HIP_TRY(hipMalloc(&ctx->d_max_ranges, ctx->max_ranges.size() * sizeof(MaxRange)));
+        HIP_TRY(hipMemcpyAsync(ctx->d_max_ranges, ctx->max_ranges.data(), ctx->max_ranges.size() * sizeof(MaxRange),
                                hipMemcpyHostToDevice, ctx->stream));
     }
     HIP_TRY(hipMalloc(&ctx->d_cells, std::max<size_t>(1, cells.size()) * sizeof(ReceiverCell)));
@@ -1790,6 +2096,7 @@ int pstd_ctx_upload_plans(pstd_ctx *ctx, const pstd_plan_desc *plans, int n_plan
     std::vector<pstd_plan_domain> doms;
     std::vector<pstd_plan_segment> segs;
     std::vector<pstd_plan_receiver> rcv;
+    std::vector<int> dom_scene;
     const pstd_plan_desc &p0 = plans[0];
     for (int i = 0; i < n_plans; ++i) {
         const pstd_plan_desc &p = plans[i];
@@ -1802,7 +2109,10 @@ int pstd_ctx_upload_plans(pstd_ctx *ctx, const pstd_plan_desc *plans, int n_plan
         }
         if (p.n_domains < 0 || p.n_segments < 0 || p.n_receivers < 0) return PSTD_ERR_INVALID;
         const int base = (int)doms.size();
-        for (int d = 0; d < p.n_domains; ++d) doms.push_back(p.domains[d]);
+        for (int d = 0; d < p.n_domains; ++d) {
+            doms.push_back(p.domains[d]);
+            dom_scene.push_back(i);
+        }
         for (int k = 0; k < p.n_segments; ++k) {
             pstd_plan_segment s = p.segments[k];
             const bool bad = s.domain < 0 || s.domain >= p.n_domains || s.side1 >= p.n_domains || s.side2 >= p.n_domains;
@@ -1832,7 +2142,7 @@ int pstd_ctx_upload_plans(pstd_ctx *ctx, const pstd_plan_desc *plans, int n_plan
     m.segments = segs.data();
     m.n_receivers = (int)rcv.size();
     m.receivers = rcv.data();
-    return upload_impl(ctx, &m);
+    return upload_impl(ctx, &m, &dom_scene);
 }
 
 int pstd_ctx_set_field(pstd_ctx *ctx, int domain, int field, const float *host, size_t count) {
@@ -1845,6 +2155,12 @@ int pstd_ctx_set_field(pstd_ctx *ctx, int domain, int field, const float *host, 
     }
     HIP_TRY(hipSetDevice(ctx->device));
     float *dst;
+    if (ctx->umax && field != PSTD_FIELD_P0) {
+        // GEMM form: the next first stage recomputes the scaling maxima from
+        // the state, which needs p0 in its explicit array
+        int rc = materialise_p0(ctx);
+        if (rc) return rc;
+    }
     if (field == PSTD_FIELD_P0) {
         int rc = materialise_p0(ctx);
         if (rc) return rc;
@@ -1976,7 +2292,8 @@ int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps) {
 }
 
 int pstd_ctx_set_form(pstd_ctx *ctx, int form) {
-    if (!ctx || (form != PSTD_FORM_FFT && form != PSTD_FORM_GEMM_F16)) return PSTD_ERR_INVALID;
+    if (!ctx || (form != PSTD_FORM_FFT && form != PSTD_FORM_GEMM_F16 && form != PSTD_FORM_GEMM_BF16))
+        return PSTD_ERR_INVALID;
     if (ctx->planned) {
         ctx->err = "the derivative form is chosen before the plan is uploaded";
         return PSTD_ERR_INVALID;
@@ -2053,6 +2370,50 @@ int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int
     if (rc) return rc;
     *mean_stage_ms = (double)ms / (6.0 * steps);
     if (launches) *launches = 6 * steps;
+    return PSTD_OK;
+}
+
+int pstd_ctx_profile_kernels(pstd_ctx *ctx, int steps, double *ms, int *launches) {
+    if (!ctx || !ctx->planned || steps <= 0 || !ms || !launches) return PSTD_ERR_INVALID;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    hipEvent_t ev[2];
+    HIP_TRY(hipEventCreate(&ev[0]));
+    HIP_TRY(hipEventCreate(&ev[1]));
+    const int graph_steps = ctx->graph_steps;
+    ctx->graph_steps = 0;  // eager launches: events cannot be recorded into the replayed graph
+    ctx->prof = true;
+    ctx->prof_events.clear();
+    int rc = PSTD_OK;
+    float total = 0;
+    if (hipEventRecord(ev[0], ctx->stream) != hipSuccess) rc = PSTD_ERR_DEVICE;
+    if (rc == PSTD_OK) rc = advance_impl(ctx, steps, nullptr);
+    if (rc == PSTD_OK && hipEventRecord(ev[1], ctx->stream) != hipSuccess) rc = PSTD_ERR_DEVICE;
+    if (rc == PSTD_OK && hipEventSynchronize(ev[1]) != hipSuccess) rc = PSTD_ERR_DEVICE;
+    if (rc == PSTD_OK && hipEventElapsedTime(&total, ev[0], ev[1]) != hipSuccess) rc = PSTD_ERR_DEVICE;
+    ctx->prof = false;
+    ctx->graph_steps = graph_steps;
+    for (int c = 0; c <= kClasses; ++c) {
+        ms[c] = 0;
+        launches[c] = 0;
+    }
+    for (auto &pe : ctx->prof_events) {
+        float t = 0;
+        if (rc == PSTD_OK && hipEventElapsedTime(&t, pe.second.first, pe.second.second) != hipSuccess) rc = PSTD_ERR_DEVICE;
+        ms[pe.first] += t;
+        launches[pe.first] += 1;
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
+    }
+    ctx->prof_events.clear();
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc) {
+        if (ctx->err.empty()) ctx->err = "profile_kernels: HIP event failure";
+        return rc;
+    }
+    ms[kClasses] = total;
+    launches[kClasses] = 6 * steps;
     return PSTD_OK;
 }
 

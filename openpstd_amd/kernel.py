@@ -219,6 +219,19 @@ class _Context:
         _check(lib().pstd_ctx_profile_stages(self.ctx, steps, C.byref(ms), C.byref(n)), self._err)
         return ms.value, n.value
 
+    KERNEL_CLASSES = ("fft_small", "fft_4096", "fft_8192", "gemm")
+
+    def profile_kernels(self, steps):
+        """Advances `steps` steps with HIP event pairs around every launch of each
+        kernel class and waits; returns {class: (GPU ms summed, launches)} and
+        "total": (GPU ms of the whole run, RK stages)."""
+        ms = (C.c_double * 5)()
+        n = (C.c_int * 5)()
+        _check(lib().pstd_ctx_profile_kernels(self.ctx, steps, ms, n), self._err)
+        out = {c: (ms[i], n[i]) for i, c in enumerate(self.KERNEL_CLASSES)}
+        out["total"] = (ms[4], n[4])
+        return out
+
     def gemm_work(self):
         """(MFMA flops, line outputs) of the GEMM-form segments per RK stage."""
         f, n = C.c_double(), C.c_longlong()

@@ -30,6 +30,7 @@ def lib():
             raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
         L = C.CDLL(LIB_PATH)
         L.oracle_set_mutation.argtypes = [C.c_int]
+        L.oracle_set_line_threads.argtypes = [C.c_int]
         L.oracle_next_2_power_int.argtypes = [C.c_int]
         L.oracle_next_2_power_float.argtypes = [C.c_float]
         L.oracle_rho_array.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]
@@ -65,6 +66,12 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def set_line_threads(n):
+    """Spreads the lines of each derivative call over n threads (fixture
+    generation only; results do not depend on n)."""
+    lib().oracle_set_line_threads(int(n))
 
 
 def set_mutation(code):

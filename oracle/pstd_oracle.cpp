@@ -50,6 +50,9 @@ static thread_local std::string g_error;
 // geometric order) instead of the bottom one.  Set only through
 // oracle_set_mutation, by tests, to show that the invariants fail on them.
 static int g_mutation = 0;
+// Threads over the lines of one spatderp3 call (1: serial); set only by fixture
+// generation through oracle_set_line_threads, never by the CPU baseline.
+static int g_line_threads = 1;
 
 enum Direction { LEFT = 0, RIGHT = 1, TOP = 2, BOTTOM = 3 };
 // kernel/core/kernel_functions.h:92-93
@@ -92,8 +95,11 @@ struct Mat {
     }
     Mat transposed() const {
         Mat t(cols, rows);
-        for (int r = 0; r < rows; ++r)
-            for (int c = 0; c < cols; ++c) t(c, r) = (*this)(r, c);
+        // 32 x 32 blocks (cache-friendly for the 4096-wide domains)
+        for (int r0 = 0; r0 < rows; r0 += 32)
+            for (int c0 = 0; c0 < cols; c0 += 32)
+                for (int r = r0; r < std::min(rows, r0 + 32); ++r)
+                    for (int c = c0; c < std::min(cols, c0 + 32); ++c) t(c, r) = (*this)(r, c);
         return t;
     }
 };
@@ -315,35 +321,41 @@ static Mat<R> spatderp3(Mat<R> p1, Mat<R> p2, Mat<R> p3, const std::vector<std::
     auto fft = fftc.get(F);
     const int outn = ct == PRESSURE ? n2 + 1 : n2 - 1;
     Mat<R> result(batch, outn);
-    std::vector<R> line(F), y(F);
-    std::vector<std::complex<R>> spec(F / 2 + 1), scratch;
     const R *wl = window.data();
     const R *wr = window.data() + wlen + 1;
     const int m1 = g_mutation == 1 ? 1 : 0;
-    for (int r = 0; r < batch; ++r) {
-        std::fill(line.begin(), line.end(), R(0));
-        if (ct == PRESSURE) {
-            if (p1.cols < wlen || p3.cols < wlen) throw std::runtime_error("window bigger than sides");
-            for (int j = 0; j < wlen; ++j) {
-                R d1 = p1(r, p1.cols - wlen + j - m1) * rho.p[2][1] + p2(r, wlen - 1 - j) * rho.p[0][0];
-                line[j] = d1 * wl[j];
-                R d3 = p3(r, j) * rho.p[3][1] + p2(r, n2 - 1 - j) * rho.p[1][0];
-                line[wlen + n2 + j] = d3 * wr[j];
+    if (ct == PRESSURE ? (p1.cols < wlen || p3.cols < wlen) : (p1.cols < wlen + 1 || p3.cols < wlen + 1))
+        throw std::runtime_error("window bigger than sides");
+    // lines are independent: g_line_threads > 1 (fixture generation) spreads
+    // them over threads with identical per-line arithmetic
+#pragma omp parallel num_threads(g_line_threads) if (g_line_threads > 1 && batch >= 64)
+    {
+        std::vector<R> line(F), y(F);
+        std::vector<std::complex<R>> spec(F / 2 + 1), scratch;
+#pragma omp for schedule(static)
+        for (int r = 0; r < batch; ++r) {
+            std::fill(line.begin(), line.end(), R(0));
+            if (ct == PRESSURE) {
+                for (int j = 0; j < wlen; ++j) {
+                    R d1 = p1(r, p1.cols - wlen + j - m1) * rho.p[2][1] + p2(r, wlen - 1 - j) * rho.p[0][0];
+                    line[j] = d1 * wl[j];
+                    R d3 = p3(r, j) * rho.p[3][1] + p2(r, n2 - 1 - j) * rho.p[1][0];
+                    line[wlen + n2 + j] = d3 * wr[j];
+                }
+            } else {
+                for (int j = 0; j < wlen; ++j) {
+                    R d1 = p1(r, p1.cols - wlen - 1 + j - m1) * rho.v[2][1] + p2(r, wlen - j) * rho.v[0][0];
+                    line[j] = d1 * wl[j];
+                    R d3 = p3(r, 1 + j) * rho.v[3][1] + p2(r, n2 - 2 - j) * rho.v[1][0];
+                    line[wlen + n2 + j] = d3 * wr[j];
+                }
             }
-        } else {
-            if (p1.cols < wlen + 1 || p3.cols < wlen + 1) throw std::runtime_error("window bigger than sides");
-            for (int j = 0; j < wlen; ++j) {
-                R d1 = p1(r, p1.cols - wlen - 1 + j - m1) * rho.v[2][1] + p2(r, wlen - j) * rho.v[0][0];
-                line[j] = d1 * wl[j];
-                R d3 = p3(r, 1 + j) * rho.v[3][1] + p2(r, n2 - 2 - j) * rho.v[1][0];
-                line[wlen + n2 + j] = d3 * wr[j];
-            }
+            for (int i = 0; i < n2; ++i) line[wlen + i] = p2(r, i);
+            fft->forward(line.data(), spec.data(), scratch);
+            for (int k = 0; k <= F / 2; ++k) spec[k] = spec[k] * derfact[k];
+            fft->inverse(spec.data(), y.data(), scratch);
+            for (int i = 0; i < outn; ++i) result(r, i) = y[wlen + i] / (R)F;
         }
-        for (int i = 0; i < n2; ++i) line[wlen + i] = p2(r, i);
-        fft->forward(line.data(), spec.data(), scratch);
-        for (int k = 0; k <= F / 2; ++k) spec[k] = spec[k] * derfact[k];
-        fft->inverse(spec.data(), y.data(), scratch);
-        for (int i = 0; i < outn; ++i) result(r, i) = y[wlen + i] / (R)F;
     }
     if (direct == Y) result = result.transposed();
     return result;
@@ -1152,6 +1164,7 @@ static std::string g_last_error;
 // Exported functions take C linkage from their declarations in pstd_oracle.h.
 
 void oracle_set_mutation(int code) { g_mutation = code; }
+void oracle_set_line_threads(int n) { g_line_threads = n > 0 ? n : 1; }
 int oracle_next_2_power_int(int n) { return next_2_power_int(n); }
 int oracle_next_2_power_float(float n) { return next_2_power_float(n); }
 

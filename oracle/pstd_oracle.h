@@ -20,6 +20,8 @@ extern "C" {
 /* Known-answer primitives (kernel/core/kernel_functions.cpp, WisdomCache.cpp) */
 /* Planted mistakes for the oracle's invariant tests (0 = none); see pstd_oracle.cpp. */
 void oracle_set_mutation(int code);
+/* Threads over the lines of one spatderp3 call (fixture generation; default 1). */
+void oracle_set_line_threads(int n);
 int oracle_next_2_power_int(int n);
 int oracle_next_2_power_float(float n);
 /* out_p / out_v: 8 values each, row-major 4x2 (kernel_functions.cpp:37-64) */

@@ -16,10 +16,13 @@
      square4096_receivers.npz (the 4096 x 4096-cell domain of the bench, 16
      seeded receivers within 200 cells of the centre source, 500 steps, FP64),
      batch64_receivers.npz (four scenes of the 64-scene batch, seed 0: 19 with
-     one domain, 56 with two, 40 with three, 50 with four; 2000 steps, FP64)
+     one domain, 56 with two, 40 with three, 50 with four; 2000 steps, FP64),
+     square4096_long_receivers.npz (the 4096 x 4096-cell domain at its
+     configured 1000 steps, 16 receivers within 400 cells of the source, FP64;
+     written every 100 steps)
 
     python tests/golden/make_fixtures.py [--kat] [--series] [--square] [--sources] [--frames] [--square4096]
-                                         [--batch64]
+                                         [--batch64] [--square4096-long]
 """
 import json
 import os
@@ -99,6 +102,23 @@ def square4096(steps=500):
     print("square4096 done", flush=True)
 
 
+def square4096_long(steps=1000, chunk=100):
+    """The 4096 x 4096-cell domain at its configured 1000 steps, 16 seeded
+    receivers within 400 cells of the centre source (FP64).  About 18 s of
+    CPU per step on 8 cores: the file is rewritten every `chunk` steps with
+    the steps done so far."""
+    from oracle import binding as ob
+    from openpstd_amd import workloads as W
+    threads = int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1))
+    conf = W.square(0, steps, n=4096, n_receivers=16, receiver_radius=400)
+    o = ob.OracleScene(conf, 64, threads=threads)
+    parts = []
+    for done in range(0, steps, chunk):
+        parts.append(o.step(min(chunk, steps - done)))
+        np.savez(os.path.join(HERE, "square4096_long_receivers.npz"), fp64=np.concatenate(parts))
+        print("square4096_long", done + len(parts[-1]), "steps", flush=True)
+
+
 BATCH64_SCENES = (19, 56, 40, 50)
 
 
@@ -145,3 +165,5 @@ if __name__ == "__main__":
         square4096()
     if "--batch64" in args:
         batch64()
+    if "--square4096-long" in args:
+        square4096_long()
