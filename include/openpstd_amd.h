@@ -198,6 +198,19 @@ int pstd_ctx_advance(pstd_ctx *ctx, int steps, float *samples);
 /* Like pstd_ctx_advance but does not synchronise and keeps the samples on the
  * device (bench and pipelined callers); pstd_ctx_synchronize waits. */
 int pstd_ctx_advance_async(pstd_ctx *ctx, int steps);
+/* Allocation-free variant of pstd_ctx_advance_async, with no counterpart in
+ * the reference: for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid it).  It calls no hipMalloc, hipFree or other
+ * allocating routine: the launches are issued one by one (no graph is
+ * instantiated), and the GEMM form's scratch -- the per-scene maxima that
+ * choose the split scaling -- lives in a file-scope device array of
+ * 3 x 2 x 1024 words per device instead of the context's buffer; more than
+ * 1024 scenes in a GEMM-form context is PSTD_ERR_INVALID.  Every call starts
+ * that array from zero and from the maxima of the current state, so results
+ * equal pstd_ctx_advance_async's from the same state (bitwise when that call
+ * also starts from host-written state).  All calls on one device share the
+ * array: calls that overlap on one device race. */
+int pstd_ctx_advance_static(pstd_ctx *ctx, int steps);
 int pstd_ctx_synchronize(pstd_ctx *ctx);
 /* Current p0 of one domain, row-major (Solver::get_pressure_vector). */
 int pstd_ctx_read_frame(pstd_ctx *ctx, int domain, float *out, size_t count);

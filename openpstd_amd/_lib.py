@@ -13,7 +13,7 @@ _lib = None
 
 EXPORTED = [
     "pstd_ctx_create", "pstd_ctx_upload_plan", "pstd_ctx_set_field", "pstd_ctx_get_field", "pstd_ctx_advance",
-    "pstd_ctx_advance_async", "pstd_ctx_synchronize", "pstd_ctx_read_frame", "pstd_ctx_steps_done",
+    "pstd_ctx_advance_async", "pstd_ctx_advance_static", "pstd_ctx_synchronize", "pstd_ctx_read_frame", "pstd_ctx_steps_done",
     "pstd_ctx_profile_stages", "pstd_ctx_profile_kernels", "pstd_ctx_cells_per_stage", "pstd_ctx_upload_plans", "pstd_ctx_set_graph_steps",
     "pstd_ctx_set_tile_order", "pstd_ctx_set_kernel_flavour", "pstd_ctx_set_tile_threads",
     "pstd_ctx_get_tile_threads", "pstd_ctx_get_tuning", "pstd_batch_prepare",
@@ -47,6 +47,7 @@ def lib():
     L.pstd_ctx_get_field.argtypes = [vp, C.c_int, C.c_int, fp, C.c_size_t]
     L.pstd_ctx_advance.argtypes = [vp, C.c_int, fp]
     L.pstd_ctx_advance_async.argtypes = [vp, C.c_int]
+    L.pstd_ctx_advance_static.argtypes = [vp, C.c_int]
     L.pstd_ctx_synchronize.argtypes = [vp]
     L.pstd_ctx_read_frame.argtypes = [vp, C.c_int, fp, C.c_size_t]
     L.pstd_ctx_steps_done.argtypes = [vp]

@@ -31,6 +31,7 @@
 #include <cstring>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -40,6 +41,13 @@
 #include "fft_line.h"
 
 using namespace pstdamd::dev;
+
+// Scratch of the allocation-free advance (pstd_ctx_advance_static): the
+// GEMM form's per-scene maxima, 3 slots x 2 classes x kStaticScenes scenes
+// (24 KiB), one copy per device (each device has its own address, resolved
+// with hipGetSymbolAddress), shared by every call on that device.
+constexpr int kStaticScenes = 1024;
+static __device__ __align__(256) unsigned scratch[3 * 2 * kStaticScenes];
 
 namespace {
 
@@ -1156,6 +1164,10 @@ struct MaxRange {
     int idx;        // scene * 2 + class
 };
 
+__global__ void fill_zero_kernel(unsigned *__restrict__ p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
 __global__ void state_max_kernel(const float *__restrict__ arena, unsigned *__restrict__ umax,
                                  const MaxRange *__restrict__ ranges, int nr, int n_scenes, int slot) {
     for (int k = blockIdx.y; k < nr; k += gridDim.y) {
@@ -1233,6 +1245,9 @@ struct pstd_ctx {
     int xcd_remap = 1;
     int lean = 0;  // stage kernel flavour (kEarly, kLate, kMid)
     float tuned_stage_ms = 0;
+    // the GEMM form's maxima in `umax` no longer describe the state (an
+    // allocation-free advance used the shared array): recompute before the next step
+    bool max_stale = false;
     // per-kernel-class timing (pstd_ctx_profile_kernels): event pairs around
     // every launch of a class while `prof` is set
     bool prof = false;
@@ -1455,6 +1470,26 @@ int materialise_p0(pstd_ctx *ctx) {
 
 extern "C" {
 
+// Device address of `scratch` per device (the array's name is a host-side
+// stand-in: kernels get this address).  Resolved once per device with that
+// device current.
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, unsigned *> g_scratch;
+
+int resolve_scratch(int device, unsigned **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it == g_scratch.end()) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return PSTD_ERR_DEVICE;
+        it = g_scratch.emplace(device, static_cast<unsigned *>(ptr)).first;
+    }
+    *out = it->second;
+    return PSTD_OK;
+}
+}  // namespace
+
 int pstd_ctx_create(int device, pstd_ctx **out) {
     if (!out) return PSTD_ERR_INVALID;
     *out = nullptr;
@@ -1483,6 +1518,12 @@ int pstd_ctx_create(int device, pstd_ctx **out) {
              hipFuncSetAttribute((const void *)stage_gemm_kernel<__bf16>, attr, kGemmLdsBytes) == hipSuccess;
     if (!lds_ok) {
         ctx->err = "cannot reserve LDS for the long-line kernels";
+    }
+    unsigned *sp = nullptr;
+    if (resolve_scratch(device, &sp) != PSTD_OK) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return PSTD_ERR_DEVICE;
     }
     *out = ctx;
     return PSTD_OK;
@@ -2234,6 +2275,12 @@ static int advance_impl(pstd_ctx *ctx, int steps, float *samples) {
     if (!ctx || !ctx->planned) return PSTD_ERR_INVALID;
     if (steps < 0) return PSTD_ERR_INVALID;
     HIP_TRY(hipSetDevice(ctx->device));
+    if (ctx->max_stale) {
+        // the next first stage recomputes the maxima from the state (explicit p0)
+        const int rc = materialise_p0(ctx);
+        if (rc) return rc;
+        ctx->max_stale = false;
+    }
     int done = 0;
     while (done < steps) {
         int chunk = std::min(steps - done, ctx->sample_cap);
@@ -2371,6 +2418,45 @@ int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int
     *mean_stage_ms = (double)ms / (6.0 * steps);
     if (launches) *launches = 6 * steps;
     return PSTD_OK;
+}
+
+int pstd_ctx_advance_static(pstd_ctx *ctx, int steps) {
+    if (!ctx || !ctx->planned || steps < 0) return PSTD_ERR_INVALID;
+    const bool gemm = !ctx->tiles_gemm.empty();
+    if (gemm && ctx->n_scenes > kStaticScenes) {
+        ctx->err = "allocation-free advance: " + std::to_string(ctx->n_scenes) + " scenes, at most " +
+                   std::to_string(kStaticScenes);
+        return PSTD_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    unsigned *sp = nullptr;
+    if (gemm) {
+        if (resolve_scratch(ctx->device, &sp) != PSTD_OK) {
+            ctx->err = "allocation-free advance: no device address for the scratch array";
+            return PSTD_ERR_DEVICE;
+        }
+        // the shared array starts from zero and the maxima of the current state
+        // (explicit p0, then the first stage's reduction), as after host writes
+        int rc = materialise_p0(ctx);
+        if (rc) return rc;
+        const int n = 3 * 2 * ctx->n_scenes;
+        hipLaunchKernelGGL(fill_zero_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, sp, n);
+        HIP_TRY(hipGetLastError());
+    }
+    unsigned *own = ctx->umax;
+    if (gemm) ctx->umax = sp;
+    int rc = PSTD_OK, done = 0;
+    while (rc == PSTD_OK && done < steps) {  // eager launches only: nothing to instantiate
+        const int chunk = std::min(steps - done, ctx->sample_cap);
+        rc = enqueue_steps(ctx, chunk);
+        if (rc == PSTD_OK) {
+            ctx->steps_done += chunk;
+            done += chunk;
+        }
+    }
+    ctx->umax = own;
+    if (gemm) ctx->max_stale = true;
+    return rc;
 }
 
 int pstd_ctx_profile_kernels(pstd_ctx *ctx, int steps, double *ms, int *launches) {

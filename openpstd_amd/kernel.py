@@ -199,6 +199,10 @@ class _Context:
     def advance_async(self, steps):
         _check(lib().pstd_ctx_advance_async(self.ctx, steps), self._err)
 
+    def advance_static(self, steps):
+        """Like advance_async without any device allocation (pstd_ctx_advance_static)."""
+        _check(lib().pstd_ctx_advance_static(self.ctx, steps), self._err)
+
     def synchronize(self):
         _check(lib().pstd_ctx_synchronize(self.ctx), self._err)
 
