@@ -102,7 +102,7 @@ def plain_fft(confs, n, steps, warmup=2):
         k.initialize_kernel(conf)
         k.set_form("fft")
         dev = k.prepare()
-        dev.advance_async(warmup)
+        dev.advance_async(warmup + steps)  # also captures the solver's hipGraph, outside the timed loop
         dev.synchronize()
         t0 = time.perf_counter()
         dev.advance_async(steps)
@@ -156,6 +156,7 @@ def secondary_lshape(steps=400, warmup=50):
     """The L-shaped scene in the FP32 FFT form: cell-stage/s and HBM roofline."""
     dev, _ = make_solver(workload("lshape", steps + warmup)[0], "fft")
     cells = dev.cells_per_stage
+    phase(f"warmup {warmup} steps, then {steps} timed steps")
     dev.advance_async(warmup)
     dev.synchronize()
     t0 = time.perf_counter()
@@ -237,6 +238,11 @@ def copy_calibration(mib):
                                            "launches": 5}}))
 
 
+def phase(msg):
+    """One progress line on stderr per bench phase (the JSON result alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,6 +279,7 @@ def main():
     steps = args.steps if args.steps is not None else DEFAULT_STEPS[args.workload]
     warmup = args.warmup if args.warmup is not None else DEFAULT_WARMUP[args.workload]
     confs, desc = workload(args.workload, warmup + 3 * steps + 4)
+    phase(f"{args.workload}: {len(confs)} scenes, form {args.form}: building plans and autotuning")
     dev, kernels = make_solver(confs, args.form)
     dev.set_graph_steps(20 if args.graph_steps is None else args.graph_steps)
     if args.tile_order != "auto":
@@ -284,6 +291,7 @@ def main():
     tuning = dev.tuning()
     cells = dev.cells_per_stage
 
+    phase(f"warmup {warmup} steps, then {steps} timed steps")
     dev.advance_async(warmup)
     dev.synchronize()
     t0 = time.perf_counter()
@@ -340,7 +348,10 @@ def main():
                    "launch_shape": {"flavour": ["early", "late", "mid"][tuning["lean"]],
                                     "xcd_order": tuning["xcd_order"], "tile_threads": tuning["tile_threads"],
                                     "autotuned": args.tile_order == "auto" and args.flavour == "auto"
-                                    and args.tile_threads == "auto"},
+                                    and args.tile_threads == "auto",
+                                    "autotune_trial_steps": tuning["trial_steps"],
+                                    "autotune_cached": tuning["cached"],
+                                    "graph_steps": 20 if args.graph_steps is None else args.graph_steps},
                    "path": ("FP32 FFT (fused LDS line derivative + RK update)" if args.form == "fft" else
                             "dense derivative matrices on MFMA, 16-bit hi+lo split operands staged from FP32 "
                             "in the kernel, FP32 accumulate, fused RK update (thin PML strips in the FP32 FFT "
@@ -349,15 +360,20 @@ def main():
         "kernel_times": kernel_times,
     }
     del dev
+    phase(f"timed: {wall / steps * 1e3:.2f} ms per step")
     if args.plain_compare and len(confs) > 1:
+        phase(f"plain FFT form, {args.plain_compare} scenes one at a time")
         plain = plain_fft(confs, args.plain_compare, steps)
         out["plain_fft"] = plain
         out["vs_plain"] = value / plain["value"]
     if args.workload == "sources256" and not args.no_accuracy:
+        phase("accuracy: 3000 steps vs the FP64 fixture")
         out["accuracy"] = accuracy(args.form)
     if not args.no_secondary and args.workload != "lshape":
+        phase("secondary: L-shape, FFT form")
         out["secondary"] = secondary_lshape()
     if not args.no_cpu:
+        phase("CPU baseline (oracle, all cores of this process)")
         out["cpu_baseline"] = cpu_baseline(confs)
     print(json.dumps(out))
 

@@ -230,9 +230,11 @@ int pstd_ctx_profile_stages(pstd_ctx *ctx, int steps, double *mean_stage_ms, int
  * the reference). */
 int pstd_ctx_profile_kernels(pstd_ctx *ctx, int steps, double ms[5], int launches[5]);
 /* Replays whole blocks of `steps` time steps as one captured hipGraph (0 = the
- * stage launches are issued one by one, the default).  The graph is captured
- * on first use after the first time step and dropped on re-upload.  Results
- * are identical either way. */
+ * stage launches are issued one by one; default 20).  The graph is captured on
+ * first use, when p0 comes from the stage outputs (the first step after an
+ * upload or a host write runs eagerly), and dropped on re-upload or a change of
+ * launch shape.  Receiver samples of a replay are copied into place on the
+ * device.  Results are identical either way. */
 int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps);
 /* Launch shape of the stage kernel (no counterpart in the reference, whose
  * FFTW plans are made with FFTW_ESTIMATE on every call,
@@ -251,13 +253,19 @@ int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps);
  *    twice the contiguous bytes of each row per wave access; 128 spreads a
  *    small scene over more CUs and needs every such line to have F <= 1024,
  *    else PSTD_ERR_INVALID).
+ * The choice is kept per (device, plan signature: form, scenes, every tile's
+ * line class, line count and direction): a later upload of a plan with the
+ * same signature on the same device takes it without trials.
  * pstd_ctx_get_tuning reports the choice and its timed GPU ms per stage,
- * pstd_ctx_get_tile_threads the block size. */
+ * pstd_ctx_get_tile_threads the block size, pstd_ctx_get_autotune the time
+ * steps the trials ran on this context (launches of the stage kernels that are
+ * not part of the caller's steps) and whether the choice came from the cache. */
 int pstd_ctx_set_tile_order(pstd_ctx *ctx, int xcd_contiguous);
 int pstd_ctx_set_kernel_flavour(pstd_ctx *ctx, int flavour);
 int pstd_ctx_set_tile_threads(pstd_ctx *ctx, int threads);
 int pstd_ctx_get_tile_threads(const pstd_ctx *ctx);
 int pstd_ctx_get_tuning(const pstd_ctx *ctx, int32_t *flavour, int32_t *xcd_contiguous, double *stage_ms);
+int pstd_ctx_get_autotune(const pstd_ctx *ctx, int32_t *trial_steps, int32_t *from_cache);
 /* Work of the GEMM-form segments in one RK stage: algorithmic MFMA flops
  * (3 split products x 2 x K x outputs x lines) and line outputs. */
 int pstd_ctx_gemm_work(const pstd_ctx *ctx, double *flops_per_stage, long long *outputs_per_stage);

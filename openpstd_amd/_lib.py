@@ -16,7 +16,7 @@ EXPORTED = [
     "pstd_ctx_advance_async", "pstd_ctx_advance_static", "pstd_ctx_synchronize", "pstd_ctx_read_frame", "pstd_ctx_steps_done",
     "pstd_ctx_profile_stages", "pstd_ctx_profile_kernels", "pstd_ctx_cells_per_stage", "pstd_ctx_upload_plans", "pstd_ctx_set_graph_steps",
     "pstd_ctx_set_tile_order", "pstd_ctx_set_kernel_flavour", "pstd_ctx_set_tile_threads",
-    "pstd_ctx_get_tile_threads", "pstd_ctx_get_tuning", "pstd_batch_prepare",
+    "pstd_ctx_get_tile_threads", "pstd_ctx_get_tuning", "pstd_ctx_get_autotune", "pstd_batch_prepare",
     "pstd_derivative_matrix", "pstd_ctx_set_form", "pstd_kernel_set_form", "pstd_ctx_gemm_work",
     "pstd_ctx_last_error", "pstd_ctx_destroy", "pstd_kernel_create", "pstd_kernel_initialize", "pstd_kernel_run",
     "pstd_kernel_get_metadata", "pstd_kernel_domain_info", "pstd_kernel_read_field", "pstd_kernel_read_pml",
@@ -60,6 +60,7 @@ def lib():
     L.pstd_ctx_set_tile_threads.argtypes = [vp, C.c_int]
     L.pstd_ctx_get_tile_threads.argtypes = [vp]
     L.pstd_ctx_get_tuning.argtypes = [vp, ip, ip, C.POINTER(C.c_double)]
+    L.pstd_ctx_get_autotune.argtypes = [vp, ip, ip]
     L.pstd_batch_prepare.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(vp)]
     L.pstd_derivative_matrix.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_size_t]

@@ -29,7 +29,6 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
-#include <cstdio>
 #include <map>
 #include <mutex>
 #include <string>
@@ -1200,6 +1199,9 @@ int next_pow2(int n) {
 
 }  // namespace
 
+// Time steps per captured hipGraph unless pstd_ctx_set_graph_steps says otherwise.
+constexpr int kDefaultGraphSteps = 20;
+
 struct pstd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1252,16 +1254,25 @@ struct pstd_ctx {
     // every launch of a class while `prof` is set
     bool prof = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_events;
-    // hipGraph of `graph_steps` whole time steps (0: launches are issued one by one)
-    int graph_steps = 0;
+    // hipGraph of `graph_steps` whole time steps (0: launches are issued one by
+    // one).  The graph writes its receiver samples into `d_gsamples`; each replay
+    // is followed by a device-side copy to its place in `d_samples`.
+    int graph_steps = kDefaultGraphSteps;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    float *d_gsamples = nullptr;
+    // autotuner bookkeeping: time steps its trials ran, and whether the launch
+    // shape came from the per-(device, plan) cache instead
+    int trial_steps = 0;
+    bool tune_cached = false;
 
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (d_gsamples) (void)hipFree(d_gsamples);
         graph_exec = nullptr;
         graph = nullptr;
+        d_gsamples = nullptr;
     }
 
     float *TN() const { return arena; }
@@ -1541,19 +1552,19 @@ void pstd_ctx_destroy(pstd_ctx *ctx) {
 const char *pstd_ctx_last_error(const pstd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 // Enqueues `chunk` time steps: 6 stage launches per step (the first stage of
-// step s also samples step s-1 into sample slot s-1) and the sampling of the
-// chunk's last step into slot chunk-1.
-static int enqueue_steps(pstd_ctx *ctx, int chunk) {
+// step s > 0 also samples step s-1 into sample slot base+s-1) and the sampling
+// of the chunk's last step into slot base+chunk-1.
+static int enqueue_steps(pstd_ctx *ctx, int chunk, int base = 0) {
     for (int s = 0; s < chunk; ++s) {
         for (int k = 0; k < 6; ++k) {
-            int rc = launch_stage(ctx, k, s - 1);
+            int rc = launch_stage(ctx, k, s == 0 ? -1 : base + s - 1);
             if (rc) return rc;
             if (k == 0) ctx->p0_explicit = false;
         }
     }
     if (ctx->nrec > 0) {
         hipLaunchKernelGGL(receiver_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->S(1), ctx->d_cells, ctx->nrec,
-                           ctx->d_samples, chunk - 1);
+                           ctx->d_samples, base + chunk - 1);
         HIP_TRY(hipGetLastError());
     }
     return PSTD_OK;
@@ -1597,7 +1608,72 @@ static int upload_tiles(pstd_ctx *ctx) {
 // order) candidate on the freshly uploaded (all-zero) state and keeps the
 // fastest.  Zero fields stay exactly zero under the update, and the arenas are
 // cleared again afterwards, so tuning leaves no trace in the results.
+// The autotuner's choices, per (device, plan signature): a later context with a
+// plan of the same launch geometry on the same device takes the choice and
+// runs no trials.
+struct TuneChoice {
+    int lean, xcd_remap, small_nt;
+    float stage_ms;
+};
+std::mutex g_tune_mu;
+std::map<std::pair<int, uint64_t>, TuneChoice> g_tune;
+
+// FNV-1a over what the launches' timing depends on: the form, the scene count,
+// every tile's line class, line count and direction, and the arena size.  Two
+// plans that hash alike launch the same grids of the same kernels.
+uint64_t plan_signature(const pstd_ctx *ctx) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](long long v) {
+        for (int i = 0; i < 8; ++i) {
+            h ^= (uint64_t)((v >> (8 * i)) & 0xff);
+            h *= 1099511628211ull;
+        }
+    };
+    mix(ctx->form);
+    mix(ctx->n_scenes);
+    mix((long long)ctx->arena_floats);
+    mix((long long)ctx->fft_segs.size());
+    for (const TileDesc &t : ctx->fft_segs) {
+        mix(t.kind);
+        mix(t.F);
+        mix(t.count);
+        mix(t.dir);
+        mix(t.vel);
+        mix(t.n_out);
+    }
+    mix((long long)ctx->tiles_extra.size());
+    mix((long long)ctx->tiles_gemm.size());
+    for (const GemmTile &g : ctx->tiles_gemm) {
+        mix(g.t.dir);
+        mix(g.t.vel);
+        mix(g.t.count);
+        mix(g.n_rows);
+        mix(g.k_pad);
+    }
+    return h;
+}
+
 static int autotune(pstd_ctx *ctx) {
+    const std::pair<int, uint64_t> key(ctx->device, plan_signature(ctx));
+    ctx->trial_steps = 0;
+    ctx->tune_cached = false;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            ctx->lean = it->second.lean;
+            ctx->xcd_remap = it->second.xcd_remap;
+            ctx->tuned_stage_ms = it->second.stage_ms;
+            ctx->tune_cached = true;
+            if (it->second.small_nt != ctx->small_nt) {
+                ctx->small_nt = it->second.small_nt;
+                const int rc = upload_tiles(ctx);
+                if (rc) return rc;
+            }
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            return PSTD_OK;
+        }
+    }
     static const int cand[6][2] = {{kLate, 1}, {kMid, 1}, {kEarly, 1}, {kLate, 0}, {kMid, 0}, {kEarly, 0}};  // (flavour, xcd order)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
@@ -1605,6 +1681,7 @@ static int autotune(pstd_ctx *ctx) {
     // one warm-up step, then GPU ms of n steps
     int n = 3;
     auto time_steps = [&](float &ms) -> int {
+        ctx->trial_steps += 1 + n;
         int rc = enqueue_steps(ctx, 1);
         if (rc == PSTD_OK && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = PSTD_ERR_DEVICE;
         if (rc == PSTD_OK) rc = enqueue_steps(ctx, n);
@@ -1663,6 +1740,10 @@ static int autotune(pstd_ctx *ctx) {
         return rc;
     }
     ctx->tuned_stage_ms = best / (6.0 * n);
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tune[key] = {ctx->lean, ctx->xcd_remap, ctx->small_nt, ctx->tuned_stage_ms};
+    }
     HIP_TRY(hipMemsetAsync(ctx->arena, 0, 4 * (size_t)ctx->arena_floats * sizeof(float), ctx->stream));
     if (ctx->umax) HIP_TRY(hipMemsetAsync(ctx->umax, 0, 3 * 2 * (size_t)ctx->n_scenes * sizeof(unsigned), ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -2254,10 +2335,15 @@ int pstd_ctx_read_frame(pstd_ctx *ctx, int domain, float *out, size_t count) {
 // from the stage outputs (after the first step), which every replay keeps.
 static int build_graph(pstd_ctx *ctx) {
     ctx->drop_graph();
+    if (ctx->nrec > 0)
+        HIP_TRY(hipMalloc(&ctx->d_gsamples, (size_t)ctx->graph_steps * ctx->nrec * sizeof(float)));
+    float *const samples = ctx->d_samples;
+    ctx->d_samples = ctx->d_gsamples;  // captured: the graph's own sample slots 0 .. graph_steps-1
     HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_steps(ctx, ctx->graph_steps);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->d_samples = samples;
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -2282,19 +2368,33 @@ static int advance_impl(pstd_ctx *ctx, int steps, float *samples) {
         ctx->max_stale = false;
     }
     int done = 0;
+    const int gs = ctx->graph_steps;
     while (done < steps) {
-        int chunk = std::min(steps - done, ctx->sample_cap);
-        const bool use_graph = ctx->graph_steps > 0 && !ctx->p0_explicit && chunk >= ctx->graph_steps;
-        if (use_graph) {
-            chunk = ctx->graph_steps;
-            if (!ctx->graph_exec) {
-                int rc = build_graph(ctx);
+        // one chunk of at most sample_cap steps fills sample slots 0 .. chunk-1:
+        // graph replays where a whole graph fits (each copies its samples into
+        // place on the device), eager launches for the rest and for a first step
+        // after host writes (the graph reads p0 from the stage outputs)
+        const int chunk = std::min(steps - done, ctx->sample_cap);
+        int pos = 0;
+        while (pos < chunk) {
+            const int left = chunk - pos;
+            if (gs > 0 && !ctx->p0_explicit && left >= gs) {
+                if (!ctx->graph_exec) {
+                    int rc = build_graph(ctx);
+                    if (rc) return rc;
+                }
+                HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+                if (samples && ctx->nrec > 0)
+                    HIP_TRY(hipMemcpyAsync(ctx->d_samples + (size_t)pos * ctx->nrec, ctx->d_gsamples,
+                                           (size_t)gs * ctx->nrec * sizeof(float), hipMemcpyDeviceToDevice,
+                                           ctx->stream));
+                pos += gs;
+            } else {
+                const int n = (gs > 0 && ctx->p0_explicit && left > gs) ? 1 : (gs > 0 && left >= gs ? gs : left);
+                int rc = enqueue_steps(ctx, n, pos);
                 if (rc) return rc;
+                pos += n;
             }
-            HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
-        } else {
-            int rc = enqueue_steps(ctx, chunk);
-            if (rc) return rc;
         }
         ctx->steps_done += chunk;
         if (samples && ctx->nrec > 0) {
@@ -2363,6 +2463,13 @@ int pstd_ctx_get_tuning(const pstd_ctx *ctx, int32_t *lean, int32_t *xcd_contigu
     if (lean) *lean = ctx->lean;
     if (xcd_contiguous) *xcd_contiguous = ctx->xcd_remap;
     if (stage_ms) *stage_ms = ctx->tuned_stage_ms;
+    return PSTD_OK;
+}
+
+int pstd_ctx_get_autotune(const pstd_ctx *ctx, int32_t *trial_steps, int32_t *from_cache) {
+    if (!ctx || !ctx->planned) return PSTD_ERR_INVALID;
+    if (trial_steps) *trial_steps = ctx->trial_steps;
+    if (from_cache) *from_cache = ctx->tune_cached ? 1 : 0;
     return PSTD_OK;
 }
 

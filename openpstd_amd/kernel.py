@@ -243,7 +243,7 @@ class _Context:
         return f.value, n.value
 
     def set_graph_steps(self, steps):
-        """Replay blocks of `steps` time steps as one captured hipGraph (0: off)."""
+        """Replay blocks of `steps` time steps as one captured hipGraph (default 20; 0: eager launches)."""
         _check(lib().pstd_ctx_set_graph_steps(self.ctx, steps), self._err)
 
     def set_tile_order(self, xcd_contiguous):
@@ -262,13 +262,17 @@ class _Context:
 
     def tuning(self):
         """The autotuner's choice: {"lean" (flavour 0 early, 1 late, 2 mid), "xcd_order", "tile_threads",
-        "stage_ms"}."""
+        "stage_ms", "trial_steps" (time steps its trials ran on this solver), "cached" (the choice came from
+        the per-(device, plan signature) cache, without trials)}."""
         lean, xcd = C.c_int32(), C.c_int32()
         ms = C.c_double()
         _check(lib().pstd_ctx_get_tuning(self.ctx, C.byref(lean), C.byref(xcd), C.byref(ms)), self._err)
         nt = lib().pstd_ctx_get_tile_threads(self.ctx)
         _check(min(nt, 0), self._err)
-        return {"lean": lean.value, "xcd_order": xcd.value, "tile_threads": nt, "stage_ms": ms.value}
+        trials, cached = C.c_int32(), C.c_int32()
+        _check(lib().pstd_ctx_get_autotune(self.ctx, C.byref(trials), C.byref(cached)), self._err)
+        return {"lean": lean.value, "xcd_order": xcd.value, "tile_threads": nt, "stage_ms": ms.value,
+                "trial_steps": trials.value, "cached": bool(cached.value)}
 
     def _get(self, i, name, shape):
         out = np.zeros(shape, dtype=np.float32)

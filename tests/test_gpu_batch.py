@@ -68,11 +68,13 @@ def test_batch_rejects_mixed_time_steps():
         P.BatchSolver([a, c])
 
 
-@pytest.mark.parametrize("graph_steps", [1, 7])
+@pytest.mark.parametrize("graph_steps", [1, 7, 20])
 def test_graph_replay_is_bitwise_eager(graph_steps):
-    """Replays with eager work, host reads and synchronisations in between."""
+    """Replays with eager work, host reads and synchronisations in between
+    (20 is the default)."""
     conf = W.lshape(1, 80)
     _, eager = single(conf)
+    eager.set_graph_steps(0)
     k, g = single(conf)
     g.set_graph_steps(graph_steps)
     for n in (1, 20, 3, 30, 26):
@@ -87,6 +89,40 @@ def test_graph_replay_is_bitwise_eager(graph_steps):
     g.set_field(0, "p0", p)
     assert np.array_equal(eager.advance(9), g.advance(9))
     assert np.array_equal(eager.field(1, "vx0"), g.field(1, "vx0"))
+
+
+def test_graph_samples_across_sample_buffer_chunks():
+    """4100 steps in one call: more than the 4096-step sample buffer, so two
+    chunks, each of an eager first piece, graph replays whose samples are
+    copied into place on the device, and an eager remainder."""
+    steps = 4100
+    conf = W.lshape(3, steps)
+    _, eager = single(conf)
+    eager.set_graph_steps(0)
+    _, g = single(conf)
+    a = eager.advance(steps)
+    b = g.advance(steps)
+    print("graph vs eager,", steps, "steps: max |diff|", np.abs(a - b).max(), "max |p|", np.abs(a).max())
+    assert np.abs(a).max() > 0
+    assert np.array_equal(a, b)
+
+
+def test_autotune_cache_per_plan():
+    """A second context of the same plan takes the first one's launch shape
+    without trials; a different plan is tuned afresh."""
+    confs = W.scene_batch(5, 10, n_scenes=4, hi=260)
+    a = P.BatchSolver(confs)
+    ta = a.tuning()
+    b = P.BatchSolver(confs)
+    tb = b.tuning()
+    print("first:", ta, "second:", tb)
+    assert tb["cached"] and tb["trial_steps"] == 0
+    assert (tb["lean"], tb["xcd_order"], tb["tile_threads"]) == (ta["lean"], ta["xcd_order"], ta["tile_threads"])
+    assert ta["cached"] or ta["trial_steps"] > 0
+    c = P.BatchSolver(confs[:3])
+    tc = c.tuning()
+    assert tc["cached"] or tc["trial_steps"] > 0
+    assert np.array_equal(a.advance(10), b.advance(10))
 
 
 @pytest.mark.parametrize("shape", [(0, 1, 256), (1, 0, 256), (1, 1, 256), (2, 0, 256), (2, 1, 256), (0, 0, 512),
@@ -162,8 +198,9 @@ def test_tile_threads_bitwise(nx, ny):
 
 def test_batch64_scenes_vs_golden():
     """The 64-scene batch at full length (2000 steps, FP32 FFT form, one launch
-    per RK stage): four of its scenes (one to four domains) against committed
-    FP64 oracle series; the same four as a GEMM-form batch."""
+    per RK stage, replayed as 20-step hipGraphs): four of its scenes (one to
+    four domains) against committed FP64 oracle series; the same four as a
+    GEMM-form batch."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "batch64_receivers.npz")
     if not os.path.exists(path):
         pytest.skip("fixture not generated (tests/golden/make_fixtures.py --batch64)")
@@ -173,8 +210,10 @@ def test_batch64_scenes_vs_golden():
     picked = sorted(int(k[len("fp64_scene"):]) for k in g.files)
     assert len(picked) == 4
     full = P.BatchSolver(confs)
+    full.set_graph_steps(20)  # the default, stated: 2000 steps as one eager step, replays and a remainder
     got_full = full.advance(steps)
     sub = P.BatchSolver([confs[i] for i in picked], form="gemm")
+    sub.set_graph_steps(20)
     got_sub = sub.advance(steps)
     for j, i in enumerate(picked):
         want = g[f"fp64_scene{i}"]
