@@ -929,6 +929,72 @@ __device__ __forceinline__ void gemm_load_u(const GemmSrc &src, const TileDesc &
     }
 }
 
+// Main-piece addressing of a tile's lines: every k of a main chunk lies in the
+// line's own block, so the byte offset splits into a per-lane part (line and
+// k-group; kOOB for lanes past the tile's lines) and a uniform part (the
+// chunk's k, passed as the buffer instruction's scalar offset).  Both parts
+// are non-negative.
+struct MainU {
+    Rsrc rA, rB;
+    int voff;  // per lane
+    int ps;    // bytes per k
+    int kofs;  // k of the first main position (the line's halo length)
+};
+
+template <int DIR, bool HASB>
+__device__ __forceinline__ MainU main_u(const GemmSrc &src, const TileDesc &td, int line, bool ok) {
+    MainU m;
+    m.rA = block_rsrc(src.A, td.m_dom, td.m_size);
+    m.rB = block_rsrc(src.B, td.m_dom + (td.m_b - td.m_a), td.m_size - (td.m_b - td.m_a));
+    m.ps = td.m_ps * 4;
+    m.kofs = td.wlen;
+    m.voff = ok ? (td.m_a + line * td.m_ls) * 4 + stage_kq<DIR>(threadIdx.x) * m.ps : kOOB;
+    return m;
+}
+
+template <int DIR, bool HASB>
+__device__ __forceinline__ void gemm_load_u_main(const MainU &m, int k0, Rsrc dotr, bool dot, UChunk &u) {
+    const int kk = k0 - m.kofs;  // >= 0 in a main chunk
+    if constexpr (DIR == 0) {
+        // X lines (element stride 1): two 16-byte loads per array
+        const int so = kk * 4;
+        const float4 x0 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rA, m.voff, so, 0));
+        const float4 x1 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rA, m.voff + 16, so, 0));
+        u.a[0] = x0.x; u.a[1] = x0.y; u.a[2] = x0.z; u.a[3] = x0.w;
+        u.a[4] = x1.x; u.a[5] = x1.y; u.a[6] = x1.z; u.a[7] = x1.w;
+        if constexpr (HASB) {
+            const float4 y0 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rB, m.voff, so, 0));
+            const float4 y1 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rB, m.voff + 16, so, 0));
+            u.b[0] = y0.x; u.b[1] = y0.y; u.b[2] = y0.z; u.b[3] = y0.w;
+            u.b[4] = y1.x; u.b[5] = y1.y; u.b[6] = y1.z; u.b[7] = y1.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) u.b[j] = 0.f;
+        }
+    } else {
+        // Y lines: a wavefront reads 64 adjacent columns of a row per k
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int so = (kk + j) * m.ps;
+            u.a[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(m.rA, m.voff, so, 0));
+            u.b[j] = HASB ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(m.rB, m.voff, so, 0)) : 0.f;
+        }
+    }
+    if (dot) {
+        const int kb = k0 + stage_kq<DIR>(threadIdx.x);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) u.dk[j] = bload(dotr, (kb + j) * 4);
+    }
+}
+
+// D loads with the chunk's k as the scalar offset: vo[0], vo[1] are the
+// per-lane offsets of rows (tid >> 2) and (tid >> 2) + 128 at k = 0.
+__device__ __forceinline__ void gemm_load_d_fast(const Rsrc &dh, const Rsrc &dl, const int (&vo)[2], int k0, DChunk &d) {
+    const int so = k0 * 2;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) d.v[i] = __builtin_amdgcn_raw_buffer_load_b128(i < 2 ? dh : dl, vo[i & 1], so, 0);
+}
+
 __device__ __forceinline__ void gemm_load_d(const Rsrc &dh, const Rsrc &dl, int n0, int k_pad, int k0, DChunk &d) {
     const int tid = threadIdx.x;
     // vectors i = 0, 1 of the hi part, 2, 3 of the lo part: rows (tid >> 2) + 128 (i & 1), piece tid & 3
@@ -1041,21 +1107,37 @@ __device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt
     // chunk c run; then chunk c+1 is converted into the other LDS buffer.
     UChunk u;
     DChunk d;
+    int vo[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) vo[i] = ((gt.n0 + (tid >> 2) + i * (kGN / 2)) * gt.k_pad + 8 * (tid & 3)) * 2;
     gemm_load_u<DIR, HASB>(r, td, L, line_ok, 0, dotr, dot, u);
-    gemm_load_d(dh, dl, gt.n0, gt.k_pad, 0, d);
+    gemm_load_d_fast(dh, dl, vo, 0, d);
     gemm_stage<E, DIR, HASB>(ulds, dlds, 0, u, d, sc, dot, gv);
     __syncthreads();
-    for (int c = 0; c < nch; ++c) {
+    // Chunks whose every k lies in the main piece, [cm0, cm1), load through
+    // MainU (no per-chunk piece logic); the halo chunks around them through
+    // the generic loader.  c: the chunk whose MFMAs run next; chunk c + 1 loads.
+    const int cm0 = (td.wlen + kGK - 1) / kGK, cm1 = max(cm0, (td.wlen + td.n_main) / kGK);
+    int c = 0;
+    auto step = [&](auto load_next) {
         const int buf = c & 1;
-        const bool next = c + 1 < nch;
-        if (next) {
-            gemm_load_u<DIR, HASB>(r, td, L, line_ok, (c + 1) * kGK, dotr, dot, u);
-            gemm_load_d(dh, dl, gt.n0, gt.k_pad, (c + 1) * kGK, d);
-        }
+        load_next((c + 1) * kGK);
+        gemm_load_d_fast(dh, dl, vo, (c + 1) * kGK, d);
         if (active) gemm_chunk<E, DIR>(ulds, dlds, buf, lg, og, acc);
-        if (next) gemm_stage<E, DIR, HASB>(ulds, dlds, buf ^ 1, u, d, sc, dot, gv);
+        gemm_stage<E, DIR, HASB>(ulds, dlds, buf ^ 1, u, d, sc, dot, gv);
         __syncthreads();
+        ++c;
+    };
+    auto generic = [&](int k0) { gemm_load_u<DIR, HASB>(r, td, L, line_ok, k0, dotr, dot, u); };
+    while (c + 1 < min(cm0, nch)) step(generic);
+    {
+        const MainU m = main_u<DIR, HASB>(r, td, td.line0 + stage_line<DIR>(tid), line_ok);
+        auto mainl = [&](int k0) { gemm_load_u_main<DIR, HASB>(m, k0, dotr, dot, u); };
+        while (c + 1 < cm1) step(mainl);
     }
+    while (c + 1 < nch) step(generic);
+    if (active) gemm_chunk<E, DIR>(ulds, dlds, c & 1, lg, og, acc);
+    __syncthreads();
     // epilogue: L = C / (2^e 2^d_exp); out = t_n - s*coef*L (Solver::update_field_values)
     const float sc_upd = a.s * td.coef, inv = ldexpf(1.f, -(e + gt.d_exp));
     const bool use_pml = a.last && td.is_pml;
