@@ -733,14 +733,17 @@ struct MfmaOps<__bf16> {
     __device__ static f32x16 mfma(v8 a, v8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
 };
 
-constexpr int kGemmThreads = 512;  // 8 waves: 2 line groups x 4 output groups of 64 x 64
+constexpr int kGemmThreads = 256;  // 4 waves: 2 line groups x 2 output groups of 64 x 64
 constexpr int kGL = 128;           // lines per tile
-constexpr int kGN = 256;           // outputs per tile
+constexpr int kGN = 128;           // outputs per tile
+// (line, 8-k group) pieces of a chunk each thread stages: 2
+constexpr int kGSlots = kGL * 32 / 8 / kGemmThreads;
 constexpr int kGK = 32;            // k per LDS chunk (two MFMA k-steps)
 constexpr int kGP = kGK + 8;       // LDS row pitch in 2-byte elements (80 B: conflict-free 16-byte operand reads)
 constexpr int kGemmLdsU = 2 * 2 * kGL * kGP;  // elements: [buffer][part][line][k]
 constexpr int kGemmLdsD = 2 * 2 * kGN * kGP;  // elements: [buffer][part][output][k]
-constexpr size_t kGemmLdsBytes = (size_t)(kGemmLdsU + kGemmLdsD) * 2;
+constexpr size_t kGemmLdsBytes = (size_t)(kGemmLdsU + kGemmLdsD) * 2;  // 80 KiB: two workgroups per CU
+static_assert(kGSlots == 2, "staging map assumes two slots per thread");
 
 struct GemmTile {
     TileDesc t;            // segment addressing; line0/count: the tile's lines (count <= 128)
@@ -778,20 +781,22 @@ __device__ __forceinline__ int gemm_scale_exp(const StageArgs &a, int scene, int
 // chunk is converted, so that nothing waits for the loads of chunk c+2 before
 // the MFMAs of chunk c; dk: the FP32 dot row's coefficients of these k.
 struct UChunk {
-    float a[8], b[8], dk[8];
+    float a[kGSlots][8], b[kGSlots][8], dk[kGSlots][8];
 };
 struct DChunk {
     u32x4 v[4];
 };
 
+// Staging map of slot q: X lines 4 threads per line (lines tid/4 and tid/4 +
+// 64); Y lines one line per lane, the k-group that of the whole wavefront
+// (groups tid/128 and tid/128 + 2).
 template <int DIR>
-__device__ __forceinline__ int stage_line(int tid) {
-    return DIR == 0 ? tid >> 2 : tid & (kGL - 1);
+__device__ __forceinline__ int stage_line(int tid, int q) {
+    return DIR == 0 ? (tid >> 2) + q * (kGemmThreads / 4) : tid & (kGL - 1);
 }
 template <int DIR>
-__device__ __forceinline__ int stage_kq(int tid) {
-    // Y: the k-group is that of the whole wavefront
-    return DIR == 0 ? (tid & 3) * 8 : __builtin_amdgcn_readfirstlane(tid >> 7) * 8;
+__device__ __forceinline__ int stage_kq(int tid, int q) {
+    return DIR == 0 ? (tid & 3) * 8 : __builtin_amdgcn_readfirstlane((tid >> 7) + q * (kGemmThreads / kGL)) * 8;
 }
 
 // raw buffer over `bytes` bytes at byte offset `byte_off` of `base`
@@ -874,9 +879,9 @@ __device__ __forceinline__ void piece_rsrc(const GemmSrc &src, const TileDesc &t
 }
 
 template <int DIR, bool HASB>
-__device__ __forceinline__ void gemm_load_u(const GemmSrc &src, const TileDesc &td, const LineAddr &L, bool ok, int k0,
-                                            Rsrc dotr, bool dot, UChunk &u) {
-    const int kb = k0 + stage_kq<DIR>(threadIdx.x);
+__device__ __forceinline__ void gemm_load_u1(const GemmSrc &src, const TileDesc &td, const LineAddr &L, bool ok, int k0,
+                                             int q, Rsrc dotr, bool dot, UChunk &u) {
+    const int kb = k0 + stage_kq<DIR>(threadIdx.x, q);
     const int piece = chunk_piece(td, L.K, k0);
     if (piece == kPieceMixed) {
         // a chunk across a piece boundary: each value from whichever block holds it
@@ -888,8 +893,8 @@ __device__ __forceinline__ void gemm_load_u(const GemmSrc &src, const TileDesc &
         for (int j = 0; j < 8; ++j) {
             const int k = kb + j;
             const int om = ok ? L.main_off(k) : kOOB, o1 = ok ? L.s1_off(k) : kOOB, o2 = ok ? L.s2_off(k) : kOOB;
-            u.a[j] = bload(mA, om) + bload(s1A, o1) + bload(s2A, o2);
-            u.b[j] = HASB ? bload(mB, om) + bload(s1B, o1) + bload(s2B, o2) : 0.f;
+            u.a[q][j] = bload(mA, om) + bload(s1A, o1) + bload(s2A, o2);
+            u.b[q][j] = HASB ? bload(mB, om) + bload(s1B, o1) + bload(s2B, o2) : 0.f;
         }
     } else {
         // one block for the whole chunk
@@ -902,15 +907,15 @@ __device__ __forceinline__ void gemm_load_u(const GemmSrc &src, const TileDesc &
         if (DIR == 0 && piece != kPieceS2) {
             // X lines inside a row: two 16-byte loads per array
             const float4 x0 = bload4(rA, off), x1 = bload4(rA, off == kOOB ? kOOB : off + 16);
-            u.a[0] = x0.x; u.a[1] = x0.y; u.a[2] = x0.z; u.a[3] = x0.w;
-            u.a[4] = x1.x; u.a[5] = x1.y; u.a[6] = x1.z; u.a[7] = x1.w;
+            u.a[q][0] = x0.x; u.a[q][1] = x0.y; u.a[q][2] = x0.z; u.a[q][3] = x0.w;
+            u.a[q][4] = x1.x; u.a[q][5] = x1.y; u.a[q][6] = x1.z; u.a[q][7] = x1.w;
             if constexpr (HASB) {
                 const float4 y0 = bload4(rB, off), y1 = bload4(rB, off == kOOB ? kOOB : off + 16);
-                u.b[0] = y0.x; u.b[1] = y0.y; u.b[2] = y0.z; u.b[3] = y0.w;
-                u.b[4] = y1.x; u.b[5] = y1.y; u.b[6] = y1.z; u.b[7] = y1.w;
+                u.b[q][0] = y0.x; u.b[q][1] = y0.y; u.b[q][2] = y0.z; u.b[q][3] = y0.w;
+                u.b[q][4] = y1.x; u.b[q][5] = y1.y; u.b[q][6] = y1.z; u.b[q][7] = y1.w;
             } else {
 #pragma unroll
-                for (int j = 0; j < 8; ++j) u.b[j] = 0.f;
+                for (int j = 0; j < 8; ++j) u.b[q][j] = 0.f;
             }
         } else {
             // Y lines (a wavefront reads a row segment per k), or the side-2
@@ -918,15 +923,22 @@ __device__ __forceinline__ void gemm_load_u(const GemmSrc &src, const TileDesc &
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int o = off == kOOB || kb + j >= L.K ? kOOB : off + j * ps;
-                u.a[j] = bload(rA, o);
-                u.b[j] = HASB ? bload(rB, o) : 0.f;
+                u.a[q][j] = bload(rA, o);
+                u.b[q][j] = HASB ? bload(rB, o) : 0.f;
             }
         }
     }
     if (dot) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) u.dk[j] = bload(dotr, (kb + j) * 4);  // zero from K on
+        for (int j = 0; j < 8; ++j) u.dk[q][j] = bload(dotr, (kb + j) * 4);  // zero from K on
     }
+}
+
+template <int DIR, bool HASB>
+__device__ __forceinline__ void gemm_load_u(const GemmSrc &src, const TileDesc &td, const LineAddr (&L)[kGSlots],
+                                            const bool (&ok)[kGSlots], int k0, Rsrc dotr, bool dot, UChunk &u) {
+#pragma unroll
+    for (int q = 0; q < kGSlots; ++q) gemm_load_u1<DIR, HASB>(src, td, L[q], ok[q], k0, q, dotr, dot, u);
 }
 
 // Main-piece addressing of a tile's lines: every k of a main chunk lies in the
@@ -936,54 +948,65 @@ __device__ __forceinline__ void gemm_load_u(const GemmSrc &src, const TileDesc &
 // are non-negative.
 struct MainU {
     Rsrc rA, rB;
-    int voff;  // per lane
+    int voff[kGSlots];  // per lane and slot
     int ps;    // bytes per k
     int kofs;  // k of the first main position (the line's halo length)
 };
 
 template <int DIR, bool HASB>
-__device__ __forceinline__ MainU main_u(const GemmSrc &src, const TileDesc &td, int line, bool ok) {
+__device__ __forceinline__ MainU main_u(const GemmSrc &src, const TileDesc &td, const bool (&ok)[kGSlots]) {
     MainU m;
     m.rA = block_rsrc(src.A, td.m_dom, td.m_size);
     m.rB = block_rsrc(src.B, td.m_dom + (td.m_b - td.m_a), td.m_size - (td.m_b - td.m_a));
     m.ps = td.m_ps * 4;
     m.kofs = td.wlen;
-    m.voff = ok ? (td.m_a + line * td.m_ls) * 4 + stage_kq<DIR>(threadIdx.x) * m.ps : kOOB;
+#pragma unroll
+    for (int q = 0; q < kGSlots; ++q) {
+        const int line = td.line0 + stage_line<DIR>(threadIdx.x, q);
+        m.voff[q] = ok[q] ? (td.m_a + line * td.m_ls) * 4 + stage_kq<DIR>(threadIdx.x, q) * m.ps : kOOB;
+    }
     return m;
 }
 
 template <int DIR, bool HASB>
 __device__ __forceinline__ void gemm_load_u_main(const MainU &m, int k0, Rsrc dotr, bool dot, UChunk &u) {
     const int kk = k0 - m.kofs;  // >= 0 in a main chunk
-    if constexpr (DIR == 0) {
-        // X lines (element stride 1): two 16-byte loads per array
-        const int so = kk * 4;
-        const float4 x0 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rA, m.voff, so, 0));
-        const float4 x1 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rA, m.voff + 16, so, 0));
-        u.a[0] = x0.x; u.a[1] = x0.y; u.a[2] = x0.z; u.a[3] = x0.w;
-        u.a[4] = x1.x; u.a[5] = x1.y; u.a[6] = x1.z; u.a[7] = x1.w;
-        if constexpr (HASB) {
-            const float4 y0 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rB, m.voff, so, 0));
-            const float4 y1 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rB, m.voff + 16, so, 0));
-            u.b[0] = y0.x; u.b[1] = y0.y; u.b[2] = y0.z; u.b[3] = y0.w;
-            u.b[4] = y1.x; u.b[5] = y1.y; u.b[6] = y1.z; u.b[7] = y1.w;
+#pragma unroll
+    for (int q = 0; q < kGSlots; ++q) {
+        if constexpr (DIR == 0) {
+            // X lines (element stride 1): two 16-byte loads per array
+            const int so = kk * 4;
+            const float4 x0 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rA, m.voff[q], so, 0));
+            const float4 x1 =
+                __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rA, m.voff[q] + 16, so, 0));
+            u.a[q][0] = x0.x; u.a[q][1] = x0.y; u.a[q][2] = x0.z; u.a[q][3] = x0.w;
+            u.a[q][4] = x1.x; u.a[q][5] = x1.y; u.a[q][6] = x1.z; u.a[q][7] = x1.w;
+            if constexpr (HASB) {
+                const float4 y0 =
+                    __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rB, m.voff[q], so, 0));
+                const float4 y1 =
+                    __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(m.rB, m.voff[q] + 16, so, 0));
+                u.b[q][0] = y0.x; u.b[q][1] = y0.y; u.b[q][2] = y0.z; u.b[q][3] = y0.w;
+                u.b[q][4] = y1.x; u.b[q][5] = y1.y; u.b[q][6] = y1.z; u.b[q][7] = y1.w;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) u.b[q][j] = 0.f;
+            }
         } else {
+            // Y lines: a wavefront reads 64 adjacent columns of a row per k
 #pragma unroll
-            for (int j = 0; j < 8; ++j) u.b[j] = 0.f;
+            for (int j = 0; j < 8; ++j) {
+                const int so = (kk + j) * m.ps;
+                u.a[q][j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(m.rA, m.voff[q], so, 0));
+                u.b[q][j] =
+                    HASB ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(m.rB, m.voff[q], so, 0)) : 0.f;
+            }
         }
-    } else {
-        // Y lines: a wavefront reads 64 adjacent columns of a row per k
+        if (dot) {
+            const int kb = k0 + stage_kq<DIR>(threadIdx.x, q);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int so = (kk + j) * m.ps;
-            u.a[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(m.rA, m.voff, so, 0));
-            u.b[j] = HASB ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(m.rB, m.voff, so, 0)) : 0.f;
+            for (int j = 0; j < 8; ++j) u.dk[q][j] = bload(dotr, (kb + j) * 4);
         }
-    }
-    if (dot) {
-        const int kb = k0 + stage_kq<DIR>(threadIdx.x);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) u.dk[j] = bload(dotr, (kb + j) * 4);
     }
 }
 
@@ -1011,22 +1034,26 @@ __device__ __forceinline__ void gemm_load_d(const Rsrc &dh, const Rsrc &dl, int 
 // the last output row (dot).
 template <typename E, int DIR, bool HASB>
 __device__ __forceinline__ void gemm_stage(E *ulds, E *dlds, int buf, const UChunk &u, const DChunk &d, float sc,
-                                           bool dot, float &gv) {
+                                           bool dot, float (&gv)[kGSlots]) {
     using V8 = typename MfmaOps<E>::v8;
-    const int tid = threadIdx.x, row = stage_line<DIR>(tid), kq = stage_kq<DIR>(tid);
+    const int tid = threadIdx.x;
     E *U0 = ulds + (buf * 2 + 0) * kGL * kGP, *U1 = ulds + (buf * 2 + 1) * kGL * kGP;
-    V8 hi, lo;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const float raw = HASB ? u.a[j] + u.b[j] : u.a[j];
-        const float x = raw * sc;
-        const E h = (E)x;
-        hi[j] = h;
-        lo[j] = (E)(x - (float)h);
-        if (dot) gv = __builtin_fmaf(raw, u.dk[j], gv);
+    for (int q = 0; q < kGSlots; ++q) {
+        const int row = stage_line<DIR>(tid, q), kq = stage_kq<DIR>(tid, q);
+        V8 hi, lo;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float raw = HASB ? u.a[q][j] + u.b[q][j] : u.a[q][j];
+            const float x = raw * sc;
+            const E h = (E)x;
+            hi[j] = h;
+            lo[j] = (E)(x - (float)h);
+            if (dot) gv[q] = __builtin_fmaf(raw, u.dk[q][j], gv[q]);
+        }
+        *reinterpret_cast<V8 *>(U0 + row * kGP + kq) = hi;
+        *reinterpret_cast<V8 *>(U1 + row * kGP + kq) = lo;
     }
-    *reinterpret_cast<V8 *>(U0 + row * kGP + kq) = hi;
-    *reinterpret_cast<V8 *>(U1 + row * kGP + kq) = lo;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int part = i >> 1, drow = (tid >> 2) + (i & 1) * (kGN / 2), c = tid & 3;
@@ -1081,7 +1108,7 @@ __device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt
     E *ulds = reinterpret_cast<E *>(gemm_lds);
     E *dlds = ulds + kGemmLdsU;
     const TileDesc &td = gt.t;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lg = wave >> 2, og = wave & 3;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lg = wave >> 1, og = wave & 1;
     const int e = gemm_scale_exp(a, td.scene, td.vel);
     const float sc = ldexpf(1.f, e);
     const GemmSrc r = gemm_src(a, td);
@@ -1090,8 +1117,11 @@ __device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt
     const bool dot = gt.dot_off >= 0;
     const Rsrc dotr = bytes_rsrc(ftab, dot ? (long long)gt.dot_off * 4 : 0, dot ? gt.K * 4 : 0);
     const int nch = gt.k_pad / kGK;
-    const bool line_ok = stage_line<DIR>(tid) < td.count;
-    const LineAddr L(td, td.line0 + stage_line<DIR>(tid), gt.K);
+    bool line_ok[kGSlots];
+#pragma unroll
+    for (int q = 0; q < kGSlots; ++q) line_ok[q] = stage_line<DIR>(tid, q) < td.count;
+    const LineAddr L[kGSlots] = {LineAddr(td, td.line0 + stage_line<DIR>(tid, 0), gt.K),
+                                 LineAddr(td, td.line0 + stage_line<DIR>(tid, 1), gt.K)};
     // waves whose lines or outputs lie outside the tile skip their MFMAs
     const bool active = 64 * lg < td.count && 64 * og < gt.n_g - gt.n0;
     f32x16 acc[2][2];
@@ -1101,7 +1131,7 @@ __device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.f;
-    float gv = 0.f;
+    float gv[kGSlots] = {0.f, 0.f};
     // One register set: the loads of chunk c+1 are issued right after the
     // barrier that makes chunk c visible, and are in flight while the MFMAs of
     // chunk c run; then chunk c+1 is converted into the other LDS buffer.
@@ -1131,7 +1161,7 @@ __device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt
     auto generic = [&](int k0) { gemm_load_u<DIR, HASB>(r, td, L, line_ok, k0, dotr, dot, u); };
     while (c + 1 < min(cm0, nch)) step(generic);
     {
-        const MainU m = main_u<DIR, HASB>(r, td, td.line0 + stage_line<DIR>(tid), line_ok);
+        const MainU m = main_u<DIR, HASB>(r, td, line_ok);
         auto mainl = [&](int k0) { gemm_load_u_main<DIR, HASB>(m, k0, dotr, dot, u); };
         while (c + 1 < cm1) step(mainl);
     }
@@ -1183,42 +1213,42 @@ __device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt
     if (dot) {
         // output n_out - 1 of every line: the FP32 dot products (no scaling)
         const int n = gt.n_g;
-        float Ldot = 0.f;
-        int line = -1;
-        if constexpr (DIR == 0) {
-            // the 4 threads of a line are adjacent lanes
-            float g = gv;
-            g += __shfl_xor(g, 1);
-            g += __shfl_xor(g, 2);
-            if ((tid & 3) == 0) {
-                line = tid >> 2;
-                Ldot = g;
-            }
-        } else {
-            // partial sums of the 4 k-groups through LDS (the ring is no longer read)
-            float *red = reinterpret_cast<float *>(gemm_lds);
-            red[(tid >> 7) * kGL + (tid & (kGL - 1))] = gv;
-            __syncthreads();
-            if (tid < kGL) {
-                line = tid;
-#pragma unroll
-                for (int k = 0; k < kGemmThreads / kGL; ++k) Ldot += red[k * kGL + tid];
-            }
-        }
-        if (line >= 0 && line < td.count) {
+        auto write_dot = [&](int line, float Ldot) {
+            if (line < 0 || line >= td.count) return;
             const int off = (td.o_f + (td.line0 + line) * td.o_ls + n * td.o_ps) * 4;
             const float tnv = bload(rt, off), pm = use_pml ? bload(rp, off) : 1.f;
             const float nv = __builtin_fmaf(-sc_upd, Ldot, tnv);
             bstore(ro, off, nv);
             if (a.last) bstore(rt, off, nv * pm);
             mx = fmaxf(mx, fabsf(nv));
+        };
+        if constexpr (DIR == 0) {
+            // the 4 threads of a line are adjacent lanes; slot q holds line tid/4 + 64 q
+#pragma unroll
+            for (int q = 0; q < kGSlots; ++q) {
+                float g = gv[q];
+                g += __shfl_xor(g, 1);
+                g += __shfl_xor(g, 2);
+                write_dot((tid & 3) == 0 ? stage_line<DIR>(tid, q) : -1, g);
+            }
+        } else {
+            // partial sums of the 4 k-groups (2 per thread) through LDS (the ring is no longer read)
+            float *red = reinterpret_cast<float *>(gemm_lds);
+            red[(tid >> 7) * kGL + (tid & (kGL - 1))] = gv[0] + gv[1];
+            __syncthreads();
+            float Ldot = 0.f;
+            if (tid < kGL) {
+#pragma unroll
+                for (int k = 0; k < kGemmThreads / kGL; ++k) Ldot += red[k * kGL + tid];
+            }
+            write_dot(tid < kGL ? tid : -1, Ldot);
         }
     }
     track_max(a, td.scene, td.vel ? 0 : 1, mx);
 }
 
 template <typename E>
-__global__ void __launch_bounds__(kGemmThreads) stage_gemm_kernel(StageArgs a, const GemmTile *__restrict__ tiles,
+__global__ void __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(2))) stage_gemm_kernel(StageArgs a, const GemmTile *__restrict__ tiles,
                                                                   const float *__restrict__ ftab) {
     // clears the slot the next stage writes (nothing reads or writes it in this stage)
     if (blockIdx.x == 0 && a.umax)
@@ -1863,17 +1893,18 @@ static double from_bf16_bits(uint16_t b) {
     return (double)f;
 }
 
-// Whether a segment runs in the GEMM form: its lines and outputs fill the
-// 128-line x 256-output tiles at least to 3/4 (thin PML strips keep the FFT form).
+// Whether a segment runs in the GEMM form: its lines and outputs fill blocks of
+// 128 lines x 256 outputs at least to 3/4 (thin PML strips keep the FFT form).
 static bool gemm_fills(int lines, int n_g) {
-    const double cap = (double)((lines + kGL - 1) / kGL * kGL) * (double)((n_g + kGN - 1) / kGN * kGN);
+    constexpr int kFillN = 256;
+    const double cap = (double)((lines + kGL - 1) / kGL * kGL) * (double)((n_g + kFillN - 1) / kFillN * kFillN);
     return 4.0 * lines * (double)n_g >= 3.0 * cap;
 }
 
 // GEMM form of one segment: its derivative matrix (built once per distinct
 // segment shape and stencil, split into 16-bit hi + lo parts of D * 2^d_exp;
 // for pressure lines the last row kept apart in FP32 for the dot product) and
-// its 128-line x 256-output tiles.
+// its 128-line x 128-output tiles.
 static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_plan_segment &s, const TileDesc &g,
                           std::vector<uint16_t> &dmat, std::vector<float> &ftab,
                           std::map<std::string, std::tuple<long long, int, int>> &dcache) {
