@@ -743,12 +743,17 @@ constexpr int kGP = kGK + 8;       // LDS row pitch in 2-byte elements (80 B: co
 constexpr int kGemmLdsU = 2 * 2 * kGL * kGP;  // elements: [buffer][part][line][k]
 constexpr int kGemmLdsD = 2 * 2 * kGN * kGP;  // elements: [buffer][part][output][k]
 constexpr size_t kGemmLdsBytes = (size_t)(kGemmLdsU + kGemmLdsD) * 2;  // 80 KiB: two workgroups per CU
+constexpr int kGemmDChunk = 2 * kGN * kGP;  // elements of one chunk's D image (both parts)
+static_assert(kGemmDChunk * 2 == 5 * 1024 * (kGemmThreads / 64), "D image: 5 KiB-loads per wave");
 static_assert(kGSlots == 2, "staging map assumes two slots per thread");
 
 struct GemmTile {
     TileDesc t;            // segment addressing; line0/count: the tile's lines (count <= 128)
-    long long d_hi, d_lo;  // D parts (2-byte elements), row-major [n_rows][k_pad]
-    int n_rows, k_pad;     // stored rows (multiple of 32) and columns (multiple of kGK) of D
+    // D of the tile's 128 outputs as LDS images (2-byte elements): chunk c's
+    // image [part hi, lo][128 outputs][kGP] (k of the chunk, then zero pad) at
+    // element d_img + c * kGemmDChunk; d_len elements in all
+    long long d_img, d_len;
+    int n_rows, k_pad;     // rows of D (multiple of kGN) and columns (multiple of kGK)
     int n0;                // first output of the tile
     int n_g;               // outputs the multiplication produces (n_out, or n_out - 1 with the dot row)
     int d_exp;             // D was stored as D * 2^d_exp
@@ -782,9 +787,6 @@ __device__ __forceinline__ int gemm_scale_exp(const StageArgs &a, int scene, int
 // the MFMAs of chunk c; dk: the FP32 dot row's coefficients of these k.
 struct UChunk {
     float a[kGSlots][8], b[kGSlots][8], dk[kGSlots][8];
-};
-struct DChunk {
-    u32x4 v[4];
 };
 
 // Staging map of slot q: X lines 4 threads per line (lines tid/4 and tid/4 +
@@ -1010,31 +1012,22 @@ __device__ __forceinline__ void gemm_load_u_main(const MainU &m, int k0, Rsrc do
     }
 }
 
-// D loads with the chunk's k as the scalar offset: vo[0], vo[1] are the
-// per-lane offsets of rows (tid >> 2) and (tid >> 2) + 128 at k = 0.
-__device__ __forceinline__ void gemm_load_d_fast(const Rsrc &dh, const Rsrc &dl, const int (&vo)[2], int k0, DChunk &d) {
-    const int so = k0 * 2;
+// Copies chunk c of the tile's D image into LDS buffer `dbuf` (20 KiB, straight
+// from memory into LDS: 20 wavefront loads of 1 KiB, 5 per wave).
+__device__ __forceinline__ void gemm_dma_d(const Rsrc &dr, void *dbuf, int c) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) d.v[i] = __builtin_amdgcn_raw_buffer_load_b128(i < 2 ? dh : dl, vo[i & 1], so, 0);
-}
-
-__device__ __forceinline__ void gemm_load_d(const Rsrc &dh, const Rsrc &dl, int n0, int k_pad, int k0, DChunk &d) {
-    const int tid = threadIdx.x;
-    // vectors i = 0, 1 of the hi part, 2, 3 of the lo part: rows (tid >> 2) + 128 (i & 1), piece tid & 3
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int row = (tid >> 2) + (i & 1) * (kGN / 2), c = tid & 3;
-        const int off = (((n0 + row) * k_pad) + k0 + 8 * c) * 2;
-        d.v[i] = __builtin_amdgcn_raw_buffer_load_b128(i < 2 ? dh : dl, off, 0, 0);
-    }
+    for (int j = 0; j < kGemmDChunk * 2 / 1024 / (kGemmThreads / 64); ++j)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(dr, (char *)dbuf + (j * (kGemmThreads / 64) + w) * 1024, 16,
+                                                 lane * 16 + w * 1024, c * kGemmDChunk * 2 + j * 4096, 0, 0);
 }
 
 // Converts and writes one chunk into LDS buffer `buf` (8 k of one line per
 // thread as one 16-byte write per part); accumulates the FP32 dot product of
 // the last output row (dot).
 template <typename E, int DIR, bool HASB>
-__device__ __forceinline__ void gemm_stage(E *ulds, E *dlds, int buf, const UChunk &u, const DChunk &d, float sc,
-                                           bool dot, float (&gv)[kGSlots]) {
+__device__ __forceinline__ void gemm_stage(E *ulds, int buf, const UChunk &u, float sc, bool dot,
+                                           float (&gv)[kGSlots]) {
     using V8 = typename MfmaOps<E>::v8;
     const int tid = threadIdx.x;
     E *U0 = ulds + (buf * 2 + 0) * kGL * kGP, *U1 = ulds + (buf * 2 + 1) * kGL * kGP;
@@ -1053,11 +1046,6 @@ __device__ __forceinline__ void gemm_stage(E *ulds, E *dlds, int buf, const UChu
         }
         *reinterpret_cast<V8 *>(U0 + row * kGP + kq) = hi;
         *reinterpret_cast<V8 *>(U1 + row * kGP + kq) = lo;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int part = i >> 1, drow = (tid >> 2) + (i & 1) * (kGN / 2), c = tid & 3;
-        *reinterpret_cast<u32x4 *>(dlds + ((buf * 2 + part) * kGN + drow) * kGP + 8 * c) = d.v[i];
     }
 }
 
@@ -1112,8 +1100,7 @@ __device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt
     const int e = gemm_scale_exp(a, td.scene, td.vel);
     const float sc = ldexpf(1.f, e);
     const GemmSrc r = gemm_src(a, td);
-    const Rsrc dh = bytes_rsrc(a.dmat, gt.d_hi * 2, gt.n_rows * gt.k_pad * 2),
-               dl = bytes_rsrc(a.dmat, gt.d_lo * 2, gt.n_rows * gt.k_pad * 2);
+    const Rsrc dr = bytes_rsrc(a.dmat, gt.d_img * 2, (int)(gt.d_len * 2));
     const bool dot = gt.dot_off >= 0;
     const Rsrc dotr = bytes_rsrc(ftab, dot ? (long long)gt.dot_off * 4 : 0, dot ? gt.K * 4 : 0);
     const int nch = gt.k_pad / kGK;
@@ -1136,13 +1123,9 @@ __device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt
     // barrier that makes chunk c visible, and are in flight while the MFMAs of
     // chunk c run; then chunk c+1 is converted into the other LDS buffer.
     UChunk u;
-    DChunk d;
-    int vo[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) vo[i] = ((gt.n0 + (tid >> 2) + i * (kGN / 2)) * gt.k_pad + 8 * (tid & 3)) * 2;
     gemm_load_u<DIR, HASB>(r, td, L, line_ok, 0, dotr, dot, u);
-    gemm_load_d_fast(dh, dl, vo, 0, d);
-    gemm_stage<E, DIR, HASB>(ulds, dlds, 0, u, d, sc, dot, gv);
+    gemm_dma_d(dr, dlds, 0);
+    gemm_stage<E, DIR, HASB>(ulds, 0, u, sc, dot, gv);
     __syncthreads();
     // Chunks whose every k lies in the main piece, [cm0, cm1), load through
     // MainU (no per-chunk piece logic); the halo chunks around them through
@@ -1152,10 +1135,11 @@ __device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt
     auto step = [&](auto load_next) {
         const int buf = c & 1;
         load_next((c + 1) * kGK);
-        gemm_load_d_fast(dh, dl, vo, (c + 1) * kGK, d);
+        // D of chunk c + 1 into the other buffer (read last before the previous barrier)
+        gemm_dma_d(dr, dlds + (buf ^ 1) * kGemmDChunk, c + 1);
         if (active) gemm_chunk<E, DIR>(ulds, dlds, buf, lg, og, acc);
-        gemm_stage<E, DIR, HASB>(ulds, dlds, buf ^ 1, u, d, sc, dot, gv);
-        __syncthreads();
+        gemm_stage<E, DIR, HASB>(ulds, buf ^ 1, u, sc, dot, gv);
+        __syncthreads();  // also waits for the D copy
         ++c;
     };
     auto generic = [&](int k0) { gemm_load_u<DIR, HASB>(r, td, L, line_ok, k0, dotr, dot, u); };
@@ -1913,7 +1897,7 @@ static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_
     const int K = sh.K(), n_out = sh.n_out();
     const bool dot_row = s.field == 0;  // n + 1 outputs: the last one as an FP32 dot product
     const int n_g = n_out - (dot_row ? 1 : 0);
-    const int k_pad = (K + kGK - 1) / kGK * kGK, n_rows = (n_g + 31) / 32 * 32;
+    const int k_pad = (K + kGK - 1) / kGK * kGK, n_rows = (n_g + kGN - 1) / kGN * kGN, nch = k_pad / kGK;
     const bool bf16 = ctx->form == PSTD_FORM_GEMM_BF16;
     char key[256];
     std::snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%a/%a/%a", sh.F, sh.table_len, sh.field, sh.w, sh.n_main,
@@ -1931,15 +1915,17 @@ static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_
         for (int i = 0; i < n_g; ++i)
             for (int k = 0; k < K; ++k) dm = std::max(dm, std::fabs(D[(size_t)i * K + k]));
         const int d_exp = dm > 0 ? 14 - std::ilogb(dm) : 0;
-        const long long hi = (long long)dmat.size(), per = (long long)n_rows * k_pad;
-        dmat.resize(dmat.size() + 2 * per, 0);
+        // LDS images: for every 128-output block b and chunk c, [hi, lo][128 rows][kGP]
+        const long long hi = (long long)dmat.size();
+        dmat.resize(dmat.size() + (size_t)(n_rows / kGN) * nch * kGemmDChunk, 0);
         for (int i = 0; i < n_g; ++i)
             for (int k = 0; k < K; ++k) {
                 const double v = std::ldexp(D[(size_t)i * K + k], d_exp);
                 const uint16_t h = bf16 ? to_bf16_bits(v) : to_f16_bits(v);
                 const double hv = bf16 ? from_bf16_bits(h) : from_f16_bits(h);
-                dmat[hi + (size_t)i * k_pad + k] = h;
-                dmat[hi + per + (size_t)i * k_pad + k] = bf16 ? to_bf16_bits(v - hv) : to_f16_bits(v - hv);
+                const size_t at = hi + ((size_t)(i / kGN) * nch + k / kGK) * kGemmDChunk + (size_t)(i % kGN) * kGP + k % kGK;
+                dmat[at] = h;
+                dmat[at + (size_t)kGN * kGP] = bf16 ? to_bf16_bits(v - hv) : to_f16_bits(v - hv);
             }
         int dot_off = -1;
         if (dot_row) {
@@ -1948,11 +1934,7 @@ static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_
         }
         it = dcache.emplace(key, std::make_tuple(hi, d_exp, dot_off)).first;
     }
-    const long long d_hi = std::get<0>(it->second), per = (long long)n_rows * k_pad;
-    if ((long long)n_rows * k_pad * 2 > 0x7fffffffLL) {
-        ctx->err = "GEMM form: derivative matrix above 2 GiB";
-        return PSTD_ERR_UNSUPPORTED;
-    }
+    const long long d_base = std::get<0>(it->second), d_len = (long long)nch * kGemmDChunk;
     ctx->gemm_flops += 3.0 * 2.0 * K * (double)n_out * (s.line_end - s.line_begin);
     ctx->gemm_cells += (long long)n_out * (s.line_end - s.line_begin);
     for (int l = s.line_begin; l < s.line_end; l += kGL) {
@@ -1961,8 +1943,7 @@ static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_
         gt.t.kind = 3;
         gt.t.line0 = l;
         gt.t.count = std::min(kGL, s.line_end - l);
-        gt.d_hi = d_hi;
-        gt.d_lo = d_hi + per;
+        gt.d_len = d_len;
         gt.n_rows = n_rows;
         gt.k_pad = k_pad;
         gt.n_g = n_g;
@@ -1970,6 +1951,7 @@ static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_
         gt.K = K;
         for (int n0 = 0; n0 < n_g; n0 += kGN) {
             gt.n0 = n0;
+            gt.d_img = d_base + (long long)(n0 / kGN) * d_len;
             gt.dot_off = n0 + kGN >= n_g ? std::get<2>(it->second) : -1;  // the last output tile
             ctx->tiles_gemm.push_back(gt);
         }
