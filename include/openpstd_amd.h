@@ -236,6 +236,12 @@ int pstd_ctx_profile_kernels(pstd_ctx *ctx, int steps, double ms[5], int launche
  * launch shape.  Receiver samples of a replay are copied into place on the
  * device.  Results are identical either way. */
 int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps);
+/* Captures and instantiates the step graph now instead of at its first use
+ * (no time step runs), so that a timed loop does not include the capture.
+ * PSTD_ERR_INVALID before the first time step after an upload or a host
+ * write; a no-op when graphs are off or already captured.  No counterpart in
+ * the reference. */
+int pstd_ctx_capture_graph(pstd_ctx *ctx);
 /* Launch shape of the stage kernel (no counterpart in the reference, whose
  * FFTW plans are made with FFTW_ESTIMATE on every call,
  * kernel/core/kernel_functions.cpp:175-186).  pstd_ctx_upload_plan(s) times

@@ -693,24 +693,26 @@ __global__ void p0_materialise_kernel(const float *__restrict__ src, float *__re
 // ------------------------------------------------------------- GEMM form
 // The windowed line derivative of a segment is a fixed linear operator: the
 // dense matrix D (n_out x K) of plan_tables.h, applied to the raw line
-// u = [side1 tail | main | side2 head] (K values).  One workgroup computes
-// 128 lines x 256 outputs of C = U D^T on MFMA and applies the RK update:
+// u = [side1 tail | main | side2 head] (K values).  One workgroup (4 waves,
+// two per CU) computes 128 lines x 128 outputs of C = U D^T on MFMA and
+// applies the RK update:
 //  * the input lines are read as FP32 straight from the stage state (p0 as
 //    px + py), scaled by the scene's power of two 2^e and split into two
 //    16-bit parts while they are staged into LDS (U = hi + lo: FP16 keeps 22
 //    bits when every scaled value is below 2^15, BF16 16 bits at any scale);
-//  * D is stored split the same way (one power of two per matrix);
+//  * D is stored split the same way (one power of two per matrix), as the
+//    LDS images of its 32-k chunks, copied straight from memory into LDS;
 //  * C = U_hi D_hi + U_hi D_lo + U_lo D_hi with v_mfma_f32_32x32x16_{f16,bf16},
-//    FP32 accumulation, over k-chunks of 32 through a double-buffered LDS
-//    ring (one workgroup barrier per chunk, the global loads of chunk c+2 in
-//    flight while chunk c is multiplied);
+//    FP32 accumulation, over k-chunks of 32 through double-buffered LDS (one
+//    workgroup barrier per chunk; the loads of chunk c+1 are in flight while
+//    chunk c is multiplied);
 //  * the scene's scale comes from the largest |value| every kernel of the
 //    previous stage wrote (a per-scene atomic maximum in three rotating
 //    slots: stage k reads slot (k+2)%3, writes slot k%3 and clears slot
 //    (k+1)%3), so no pass over the inputs precedes the multiplication;
 //  * pressure lines have n+1 outputs: the multiplication covers the first n,
 //    the last is an FP32 dot product of the raw line with D's last row,
-//    accumulated by the threads while they stage the line (its 256-wide
+//    accumulated by the threads while they stage the line (its 128-wide
 //    output tiles then fit n exactly instead of wasting most of a tile).
 // X lines put U on the MFMA A side (rows of C = lines, columns = outputs),
 // Y lines put D there (rows = outputs, columns = lines), so that consecutive
@@ -2519,6 +2521,18 @@ int pstd_ctx_synchronize(pstd_ctx *ctx) {
 }
 
 int pstd_ctx_steps_done(const pstd_ctx *ctx) { return ctx ? ctx->steps_done : -1; }
+
+int pstd_ctx_capture_graph(pstd_ctx *ctx) {
+    if (!ctx || !ctx->planned) return PSTD_ERR_INVALID;
+    if (ctx->graph_steps <= 0 || ctx->graph_exec) return PSTD_OK;
+    if (ctx->p0_explicit) {
+        ctx->err = "the step graph is captured once p0 comes from the stage outputs (after a time step)";
+        return PSTD_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return build_graph(ctx);
+}
 
 int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps) {
     if (!ctx || steps < 0) return PSTD_ERR_INVALID;

@@ -246,6 +246,10 @@ class _Context:
         """Replay blocks of `steps` time steps as one captured hipGraph (default 20; 0: eager launches)."""
         _check(lib().pstd_ctx_set_graph_steps(self.ctx, steps), self._err)
 
+    def capture_graph(self):
+        """Capture the step graph now (after a first time step), outside a timed loop."""
+        _check(lib().pstd_ctx_capture_graph(self.ctx), self._err)
+
     def set_tile_order(self, xcd_contiguous):
         _check(lib().pstd_ctx_set_tile_order(self.ctx, int(bool(xcd_contiguous))), self._err)
 

@@ -77,10 +77,14 @@ def test_graph_replay_is_bitwise_eager(graph_steps):
     eager.set_graph_steps(0)
     k, g = single(conf)
     g.set_graph_steps(graph_steps)
+    with pytest.raises(P.PSTDError):
+        g.capture_graph()  # p0 still explicit: no graph before the first step
     for n in (1, 20, 3, 30, 26):
         print("advance", n, "graph", graph_steps)
         a = eager.advance(n)
         b = g.advance(n)
+        if n == 1:
+            g.capture_graph()  # captured ahead of its first replay
         assert np.array_equal(a, b)
         for d in range(k.domain_count()):
             assert np.array_equal(eager.field(d, "p0"), g.field(d, "p0")), d
