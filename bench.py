@@ -263,6 +263,8 @@ def main():
                     help="FFT stage-kernel flavour (default: the autotuner's choice)")
     ap.add_argument("--tile-threads", default="auto", choices=["auto", "128", "256", "512"],
                     help="block size for lines of FFT length <= 2048 (default: the autotuner's choice)")
+    ap.add_argument("--overlap", action="store_true",
+                    help="GEMM form: run a stage's FFT-form tiles on a second stream beside the GEMM kernel")
     ap.add_argument("--plain-compare", type=int, default=8, metavar="N",
                     help="batches: time the first N scenes one at a time in the plain FFT form (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -285,6 +287,8 @@ def main():
     phase(f"{args.workload}: {len(confs)} scenes, form {args.form}: building plans and autotuning")
     dev, kernels = make_solver(confs, args.form)
     dev.set_graph_steps(20 if args.graph_steps is None else args.graph_steps)
+    if args.overlap:
+        dev.set_stage_overlap(True)
     if args.tile_order != "auto":
         dev.set_tile_order(args.tile_order == "xcd")
     if args.flavour != "auto":

@@ -242,6 +242,14 @@ int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps);
  * write; a no-op when graphs are off or already captured.  No counterpart in
  * the reference. */
 int pstd_ctx_capture_graph(pstd_ctx *ctx);
+/* on = 1: in a stage with GEMM-form tiles, the FFT-form tiles (thin
+ * segments, PML strips, passthrough and receiver tiles) run on a second
+ * stream of the context, concurrently with the GEMM kernel, forked after and
+ * joined before the neighbouring stages.  Default 0: issued in order on one
+ * stream (the GEMM kernel occupies every CU's LDS, so concurrency gains
+ * nothing measurable).  Results are identical either way.  No counterpart in
+ * the reference. */
+int pstd_ctx_set_stage_overlap(pstd_ctx *ctx, int on);
 /* Launch shape of the stage kernel (no counterpart in the reference, whose
  * FFTW plans are made with FFTW_ESTIMATE on every call,
  * kernel/core/kernel_functions.cpp:175-186).  pstd_ctx_upload_plan(s) times

@@ -15,6 +15,7 @@ EXPORTED = [
     "pstd_ctx_create", "pstd_ctx_upload_plan", "pstd_ctx_set_field", "pstd_ctx_get_field", "pstd_ctx_advance",
     "pstd_ctx_advance_async", "pstd_ctx_advance_static", "pstd_ctx_synchronize", "pstd_ctx_read_frame", "pstd_ctx_steps_done",
     "pstd_ctx_profile_stages", "pstd_ctx_profile_kernels", "pstd_ctx_cells_per_stage", "pstd_ctx_upload_plans", "pstd_ctx_set_graph_steps", "pstd_ctx_capture_graph",
+    "pstd_ctx_set_stage_overlap",
     "pstd_ctx_set_tile_order", "pstd_ctx_set_kernel_flavour", "pstd_ctx_set_tile_threads",
     "pstd_ctx_get_tile_threads", "pstd_ctx_get_tuning", "pstd_ctx_get_autotune", "pstd_batch_prepare",
     "pstd_derivative_matrix", "pstd_ctx_set_form", "pstd_kernel_set_form", "pstd_ctx_gemm_work",
@@ -56,6 +57,7 @@ def lib():
     L.pstd_ctx_upload_plans.argtypes = [vp, vp, C.c_int]
     L.pstd_ctx_set_graph_steps.argtypes = [vp, C.c_int]
     L.pstd_ctx_capture_graph.argtypes = [vp]
+    L.pstd_ctx_set_stage_overlap.argtypes = [vp, C.c_int]
     L.pstd_ctx_set_tile_order.argtypes = [vp, C.c_int]
     L.pstd_ctx_set_kernel_flavour.argtypes = [vp, C.c_int]
     L.pstd_ctx_set_tile_threads.argtypes = [vp, C.c_int]

@@ -1303,6 +1303,14 @@ constexpr int kDefaultGraphSteps = 20;
 struct pstd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // with `overlap`, GEMM-form stages run their FFT-form tiles on `side`,
+    // concurrently with the GEMM kernel on `stream` (fork and join through the
+    // two events; both streams at the default priority).  Off by default: the
+    // GEMM kernel fills every CU's LDS, so the two only trade places (measured
+    // +1 % on 64 scenes, 0 on 256 sources, -4 % on one 4096^2 domain)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool overlap = false;
     std::string err;
     bool planned = false;
     // plan
@@ -1431,7 +1439,7 @@ constexpr int kClassSmall = 0, kClass4096 = 1, kClass8192 = 2, kClassGemm = 3, k
 
 // Brackets the launches issued by `launch` with an event pair when profiling.
 template <class Fn>
-void timed_launch(pstd_ctx *ctx, int cls, Fn launch) {
+void timed_launch(pstd_ctx *ctx, hipStream_t st, int cls, Fn launch) {
     if (!ctx->prof) {
         launch();
         return;
@@ -1441,9 +1449,9 @@ void timed_launch(pstd_ctx *ctx, int cls, Fn launch) {
         launch();
         return;
     }
-    (void)hipEventRecord(e0, ctx->stream);
+    (void)hipEventRecord(e0, st);
     launch();
-    (void)hipEventRecord(e1, ctx->stream);
+    (void)hipEventRecord(e1, st);
     ctx->prof_events.push_back({cls, {e0, e1}});
 }
 
@@ -1496,37 +1504,46 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
     }
     const int ns = (int)ctx->tiles_small.size(), n4 = (int)ctx->tiles_4096.size(), n8 = (int)ctx->tiles_8192.size();
     const int fl = ctx->lean;
-    if (ns) timed_launch(ctx, kClassSmall, [&] {
+    // FFT-form tiles beside a GEMM launch: on the side stream, forked after
+    // everything issued so far and joined before anything issued later
+    const bool fork = ng && (ns || n4 || n8) && ctx->overlap && ctx->side;
+    const hipStream_t fs = fork ? ctx->side : ctx->stream;
+    if (fork) {
+        HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+    }
+    if (ns) timed_launch(ctx, fs, kClassSmall, [&] {
         a.tiles = ctx->d_tiles;
         if (ctx->small_nt == 512) {
-            if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<512, kLate>), dim3(ns), dim3(512), 0, ctx->stream, a);
-            else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<512, kMid>), dim3(ns), dim3(512), 0, ctx->stream, a);
-            else hipLaunchKernelGGL((stage_small_kernel<512, kEarly>), dim3(ns), dim3(512), 0, ctx->stream, a);
+            if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<512, kLate>), dim3(ns), dim3(512), 0, fs, a);
+            else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<512, kMid>), dim3(ns), dim3(512), 0, fs, a);
+            else hipLaunchKernelGGL((stage_small_kernel<512, kEarly>), dim3(ns), dim3(512), 0, fs, a);
         } else if (ctx->small_nt == 128) {
-            if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<128, kLate>), dim3(ns), dim3(128), 0, ctx->stream, a);
-            else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<128, kMid>), dim3(ns), dim3(128), 0, ctx->stream, a);
-            else hipLaunchKernelGGL((stage_small_kernel<128, kEarly>), dim3(ns), dim3(128), 0, ctx->stream, a);
+            if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<128, kLate>), dim3(ns), dim3(128), 0, fs, a);
+            else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<128, kMid>), dim3(ns), dim3(128), 0, fs, a);
+            else hipLaunchKernelGGL((stage_small_kernel<128, kEarly>), dim3(ns), dim3(128), 0, fs, a);
         } else {
-            if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<256, kLate>), dim3(ns), dim3(256), 0, ctx->stream, a);
-            else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<256, kMid>), dim3(ns), dim3(256), 0, ctx->stream, a);
-            else hipLaunchKernelGGL((stage_small_kernel<256, kEarly>), dim3(ns), dim3(256), 0, ctx->stream, a);
+            if (fl == kLate) hipLaunchKernelGGL((stage_small_kernel<256, kLate>), dim3(ns), dim3(256), 0, fs, a);
+            else if (fl == kMid) hipLaunchKernelGGL((stage_small_kernel<256, kMid>), dim3(ns), dim3(256), 0, fs, a);
+            else hipLaunchKernelGGL((stage_small_kernel<256, kEarly>), dim3(ns), dim3(256), 0, fs, a);
         }
     });
-    if (n4) timed_launch(ctx, kClass4096, [&] {
+    if (n4) timed_launch(ctx, fs, kClass4096, [&] {
         a.tiles = ctx->d_tiles + ns;
         size_t lds = 2 * sizeof(float2) * LdsMap<4096, 1, 0>::SIZE;
-        if (fl == kLate) hipLaunchKernelGGL((stage_big_kernel<4096, kLate>), dim3(n4), dim3(512), lds, ctx->stream, a);
-        else if (fl == kMid) hipLaunchKernelGGL((stage_big_kernel<4096, kMid>), dim3(n4), dim3(512), lds, ctx->stream, a);
-        else hipLaunchKernelGGL((stage_big_kernel<4096, kEarly>), dim3(n4), dim3(512), lds, ctx->stream, a);
+        if (fl == kLate) hipLaunchKernelGGL((stage_big_kernel<4096, kLate>), dim3(n4), dim3(512), lds, fs, a);
+        else if (fl == kMid) hipLaunchKernelGGL((stage_big_kernel<4096, kMid>), dim3(n4), dim3(512), lds, fs, a);
+        else hipLaunchKernelGGL((stage_big_kernel<4096, kEarly>), dim3(n4), dim3(512), lds, fs, a);
     });
-    if (n8) timed_launch(ctx, kClass8192, [&] {
+    if (n8) timed_launch(ctx, fs, kClass8192, [&] {
         a.tiles = ctx->d_tiles + ns + n4;
         size_t lds = 2 * sizeof(float2) * LdsMap<8192, 1, 0>::SIZE;
-        if (fl == kLate) hipLaunchKernelGGL((stage_big_kernel<8192, kLate>), dim3(n8), dim3(1024), lds, ctx->stream, a);
-        else if (fl == kMid) hipLaunchKernelGGL((stage_big_kernel<8192, kMid>), dim3(n8), dim3(1024), lds, ctx->stream, a);
-        else hipLaunchKernelGGL((stage_big_kernel<8192, kEarly>), dim3(n8), dim3(1024), lds, ctx->stream, a);
+        if (fl == kLate) hipLaunchKernelGGL((stage_big_kernel<8192, kLate>), dim3(n8), dim3(1024), lds, fs, a);
+        else if (fl == kMid) hipLaunchKernelGGL((stage_big_kernel<8192, kMid>), dim3(n8), dim3(1024), lds, fs, a);
+        else hipLaunchKernelGGL((stage_big_kernel<8192, kEarly>), dim3(n8), dim3(1024), lds, fs, a);
     });
-    if (ng) timed_launch(ctx, kClassGemm, [&] {
+    if (fork) HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
+    if (ng) timed_launch(ctx, ctx->stream, kClassGemm, [&] {
         if (ctx->form == PSTD_FORM_GEMM_BF16)
             hipLaunchKernelGGL(stage_gemm_kernel<__bf16>, dim3(ng), dim3(kGemmThreads), kGemmLdsBytes, ctx->stream, a,
                                ctx->d_gemm, ctx->ftab);
@@ -1534,6 +1551,7 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
             hipLaunchKernelGGL(stage_gemm_kernel<_Float16>, dim3(ng), dim3(kGemmThreads), kGemmLdsBytes, ctx->stream, a,
                                ctx->d_gemm, ctx->ftab);
     });
+    if (fork) HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx->err = std::string("stage launch: ") + hipGetErrorString(e);
@@ -1611,6 +1629,11 @@ int pstd_ctx_create(int device, pstd_ctx **out) {
         delete ctx;
         return PSTD_ERR_DEVICE;
     }
+    // the side stream and its events (pstd_ctx_set_stage_overlap refuses to
+    // turn the overlap on if one of them could not be created)
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) ctx->side = nullptr;
+    if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess) ctx->ev_fork = nullptr;
+    if (hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) ctx->ev_join = nullptr;
     // the long-line classes need more than 64 KiB of LDS per workgroup
     const size_t lds8 = 2 * sizeof(float2) * LdsMap<8192, 1, 0>::SIZE, lds4 = 2 * sizeof(float2) * LdsMap<4096, 1, 0>::SIZE;
     const hipFuncAttribute attr = hipFuncAttributeMaxDynamicSharedMemorySize;
@@ -1643,6 +1666,10 @@ void pstd_ctx_destroy(pstd_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->release();
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -2521,6 +2548,19 @@ int pstd_ctx_synchronize(pstd_ctx *ctx) {
 }
 
 int pstd_ctx_steps_done(const pstd_ctx *ctx) { return ctx ? ctx->steps_done : -1; }
+
+int pstd_ctx_set_stage_overlap(pstd_ctx *ctx, int on) {
+    if (!ctx) return PSTD_ERR_INVALID;
+    if (on && !(ctx->side && ctx->ev_fork && ctx->ev_join)) {
+        ctx->err = "no side stream for overlapped stages";
+        return PSTD_ERR_DEVICE;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graph();
+    ctx->overlap = on != 0;
+    return PSTD_OK;
+}
 
 int pstd_ctx_capture_graph(pstd_ctx *ctx) {
     if (!ctx || !ctx->planned) return PSTD_ERR_INVALID;

@@ -246,6 +246,10 @@ class _Context:
         """Replay blocks of `steps` time steps as one captured hipGraph (default 20; 0: eager launches)."""
         _check(lib().pstd_ctx_set_graph_steps(self.ctx, steps), self._err)
 
+    def set_stage_overlap(self, on):
+        """GEMM-form stages: FFT-form tiles on a second stream beside the GEMM kernel, or in order (default)."""
+        _check(lib().pstd_ctx_set_stage_overlap(self.ctx, int(bool(on))), self._err)
+
     def capture_graph(self):
         """Capture the step graph now (after a first time step), outside a timed loop."""
         _check(lib().pstd_ctx_capture_graph(self.ctx), self._err)

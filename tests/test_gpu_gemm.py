@@ -123,6 +123,26 @@ def test_gemm_batch_is_bitwise_single():
             assert np.array_equal(b.field(i, d, "p0"), dev.field(d, "p0")), (i, d)
 
 
+@pytest.mark.parametrize("graph_steps", [0, 20])
+def test_stage_overlap_is_bitwise(graph_steps):
+    """The FFT-form tiles of a GEMM-form stage on the side stream, beside the
+    GEMM kernel, give the same bits as the two issued in order (eagerly and
+    replayed as graphs); a run longer than one graph."""
+    confs = [gemm_pair(45, seed=s) for s in range(2)] + W.scene_batch(11, 45, n_scenes=2, hi=300)
+    runs = []
+    for overlap in (True, False):
+        b = P.BatchSolver(confs, form="gemm")
+        b.set_graph_steps(graph_steps)
+        b.set_stage_overlap(overlap)
+        got = b.advance(45)
+        runs.append((got, [b.field(i, 0, "p0") for i in range(len(confs))]))
+        print("overlap", overlap, "graph", graph_steps, "max |p|", np.abs(got).max())
+    assert np.abs(runs[0][0]).max() > 0
+    assert np.array_equal(runs[0][0], runs[1][0])
+    for x, y in zip(runs[0][1], runs[1][1]):
+        assert np.array_equal(x, y)
+
+
 def test_square1024_forms_vs_golden():
     """A 1024 x 1024-cell domain (GEMM tiles filled in both directions) against
     the committed FP64 oracle series (300 steps, 16 receivers near the source),
