@@ -93,6 +93,24 @@ def sources(n_sources=2, steps=3000):
     np.savez(os.path.join(HERE, "sources1024_receivers.npz"), **out)
 
 
+def sources_more(n_total, steps=3000):
+    """Adds FP64 series of the next seeded source positions of the same
+    geometry (sources 0 .. n_total-1 of one seed: the first ones are those
+    already stored) to sources1024_receivers.npz, saving after each (about
+    50 CPU minutes per source on 8 cores)."""
+    from oracle import binding as ob
+    from openpstd_amd import workloads as W
+    threads = int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1))
+    path = os.path.join(HERE, "sources1024_receivers.npz")
+    out = dict(np.load(path)) if os.path.exists(path) else {}
+    for i, conf in enumerate(W.source_batch(0, steps, n_sources=n_total)):
+        if f"fp64_source{i}" in out:
+            continue
+        out[f"fp64_source{i}"] = ob.OracleScene(conf, 64, threads=threads).step(steps)
+        np.savez(path, **out)
+        print("source", i, "done", flush=True)
+
+
 def square4096(steps=500):
     from oracle import binding as ob
     from openpstd_amd import workloads as W
@@ -167,3 +185,5 @@ if __name__ == "__main__":
         batch64()
     if "--square4096-long" in args:
         square4096_long()
+    if "--sources-more" in args:
+        sources_more(int(args[args.index("--sources-more") + 1]))
