@@ -332,7 +332,9 @@ def main():
                     "kernel": f"the stage's FFT-form launches (dominant: {kname}, mean event-timed launch "
                               f"{mean_ms * 1e3:.1f} us); 48 B per cell of every non-rigid domain over the stage time",
                     "mean_launch_us": mean_ms * 1e3, "stage_us": stage_ms * 1e3,
-                    "bytes_per_stage": BYTES_PER_CELL_STAGE * cells}
+                    "bytes_per_stage": BYTES_PER_CELL_STAGE * cells,
+                    "limiter": "latency and issue, not HBM bandwidth: one line pair per tile, its gather, "
+                               "transform passes and update in sequence (counters in profiles/)"}
     else:
         flops, outs = dev.gemm_work()
         g_ms, g_n = prof["gemm"]
@@ -344,6 +346,9 @@ def main():
                     "kernel": "stage_gemm_kernel (one launch per RK stage: every GEMM-form tile of every scene)",
                     "mean_launch_us": mean_ms * 1e3, "launches_timed": g_n, "flops_per_launch": flops,
                     "gemm_line_outputs_per_launch": outs,
+                    "limiter": "latency and issue: the kernel's MFMA loop alone (operands from LDS) runs at 86 % of "
+                               "the peak; load latency, the FP32->16-bit split with its LDS staging and the epilogue "
+                               "make up the rest (profiles/stage_gemm_kernel_sources256_counters.md)",
                     "whole_stage": {"stage_us": stage_ms * 1e3,
                                     "achieved_tflops": flops / (stage_ms * 1e-3) / 1e12,
                                     "frac": flops / (stage_ms * 1e-3) / 1e12 / MFMA_PEAK_TFLOPS}}
