@@ -83,6 +83,7 @@ struct TileDesc {
     int line0, count;
     int wlen, wlog, n_main, n_out, vel, dir;  // wlog: log2 of the halo slot (2^wlog >= wlen)
     int h_off, tw_off, win_off;
+    int tw2_off;  // F = 8192: the 4096-point twiddle table (its two halves run as 4096-point transforms)
     long long m_dom, s1_dom, s2_dom;
     int m_size, s1_size, s2_size;
     int m_a, m_b, m_ls, m_ps;
@@ -584,6 +585,252 @@ __device__ __forceinline__ void derivative_tile(const StageArgs &a, const TileDe
     else tile_layout<NT, FMIN, FMAX, FL, 1>(a, td, buf0, buf1);
 }
 
+// F = 8192 lines as two 4096-point transforms.  One line pair per 512-thread
+// block; thread t holds positions n = t + 512 m (m = 0..15) of both lines (as
+// one complex value z, line 0 real, line 1 imaginary).  One radix-2
+// decimation-in-frequency step in registers: with W = e^{-2 pi i / 8192},
+// a[n] = z[n] + z[n+4096] and b[n] = (z[n] - z[n+4096]) W^n are the inputs of
+// the even and odd bins, Z[2k] = FFT4096(a)[k], Z[2k+1] = FFT4096(b)[k].  After
+// the filter H (8192 bins, 1/8192 folded in) the inverse recombines in the same
+// threads: y[n] = e[n] + W^-n o[n], y[n+4096] = e[n] - W^-n o[n], where
+// e = IFFT4096(H[2k] Z[2k]) and o = IFFT4096(H[2k+1] Z[2k+1]).  Half the LDS
+// of one 8192-point transform per block: two blocks share a CU.
+constexpr int kSplitT = 512;
+constexpr size_t kSplitLds = 2 * sizeof(float2) * LdsMap<4096, 1, 0>::SIZE;  // 72 KiB (the raw staging fits)
+
+struct SplitGather {
+    float ra[16][2], rb[16][2];
+    float h1a[2], h1b[2], h2a[2], h2b[2];
+    int hpos1, hpos2;
+    float win;
+};
+
+// raw staging of the pair in LDS (floats): rows line-major at pitch 9F/8,
+// columns position-major with the two lines adjacent
+template <int LAYOUT>
+__device__ __forceinline__ int split_stage_at(int ll, int pos) {
+    return LAYOUT == 0 ? ll * (9 * 8192 / 8) + pos : ll + 2 * pos;
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ void split_gather_issue(const StageArgs &a, const TileDesc &td, int t, SplitGather &R) {
+    const int w = td.wlen, nm = td.n_main, vel = td.vel, count = td.count;
+    __builtin_assume(w > 0 && w <= 32);
+    R.win = (int)threadIdx.x <= 2 * w ? a.ftab[td.win_off + threadIdx.x] : 0.f;
+    const float *A = vel ? a.vsrc : a.p0a;
+    const bool hasB = !vel && a.p0b != nullptr;
+    const float *B = hasB ? a.p0b : A;
+    const Rsrc rmA = block_rsrc(A, td.m_dom, td.m_size);
+    const Rsrc rmB = block_rsrc(B, td.m_dom + (td.m_b - td.m_a), td.m_size - (td.m_b - td.m_a));
+    const int pstep = td.m_ps * 4;
+    const int lbase = (td.m_a + td.line0 * td.m_ls) * 4 + (t - w) * pstep, lstep = td.m_ls * 4,
+              mstep = kSplitT * pstep;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int e = t + m * kSplitT - w;
+        const bool ok = e >= 0 && e < nm;
+        const int o0 = ok ? lbase + m * mstep : kOOB;
+        const int o1 = ok && 1 < count ? lbase + lstep + m * mstep : kOOB;
+        if constexpr (LAYOUT == 1) {
+            // columns: the two lines are adjacent floats (an odd last line reads
+            // the block's spare float after its partner; dropped below)
+            const float2 x = bload2(rmA, o0);
+            const float2 y = hasB ? bload2(rmB, o0) : make_float2(0.f, 0.f);
+            R.ra[m][0] = x.x;
+            R.ra[m][1] = x.y;
+            R.rb[m][0] = y.x;
+            R.rb[m][1] = y.y;
+        } else {
+            R.ra[m][0] = bload(rmA, o0);
+            R.ra[m][1] = bload(rmA, o1);
+            R.rb[m][0] = hasB ? bload(rmB, o0) : 0.f;
+            R.rb[m][1] = hasB ? bload(rmB, o1) : 0.f;
+        }
+    }
+    // halos: 2 w values per line from the side blocks; rows: item (line, j),
+    // columns: item j with both lines in one 8-byte load
+    const Rsrc r1A = block_rsrc(A, td.s1_dom, td.s1_size);
+    const Rsrc r1B = block_rsrc(B, td.s1_dom + (td.s1_b - td.s1_a), td.s1_size - (td.s1_b - td.s1_a));
+    const Rsrc r2A = block_rsrc(A, td.s2_dom, td.s2_size);
+    const Rsrc r2B = block_rsrc(B, td.s2_dom + (td.s2_b - td.s2_a), td.s2_size - (td.s2_b - td.s2_a));
+    const int i = threadIdx.x;
+    const int ll = LAYOUT == 0 ? i >> td.wlog : 0, j = LAYOUT == 0 ? i & ((1 << td.wlog) - 1) : i;
+    const bool item = LAYOUT == 0 ? (ll < 2 && j < w) : j < w;
+    const bool ok = item && ll < count;
+    const int line = td.line0 + ll;
+    const int o1 = ok && td.s1_len >= 0
+                       ? (td.s1_a + (line + td.s1_line) * td.s1_ls + (td.s1_len - w - vel + j) * td.s1_ps) * 4
+                       : kOOB;
+    const int o2 = ok && td.s2_len >= 0 ? (td.s2_a + (line + td.s2_line) * td.s2_ls + (j + vel) * td.s2_ps) * 4 : kOOB;
+    if constexpr (LAYOUT == 1) {
+        const float2 x1 = bload2(r1A, o1), x2 = bload2(r2A, o2);
+        const float2 y1 = hasB ? bload2(r1B, o1) : make_float2(0.f, 0.f);
+        const float2 y2 = hasB ? bload2(r2B, o2) : make_float2(0.f, 0.f);
+        R.h1a[0] = x1.x; R.h1a[1] = x1.y; R.h2a[0] = x2.x; R.h2a[1] = x2.y;
+        R.h1b[0] = y1.x; R.h1b[1] = y1.y; R.h2b[0] = y2.x; R.h2b[1] = y2.y;
+    } else {
+        R.h1a[0] = bload(r1A, o1);
+        R.h2a[0] = bload(r2A, o2);
+        R.h1b[0] = hasB ? bload(r1B, o1) : 0.f;
+        R.h2b[0] = hasB ? bload(r2B, o2) : 0.f;
+        R.h1a[1] = R.h2a[1] = R.h1b[1] = R.h2b[1] = 0.f;
+    }
+    R.hpos1 = item ? split_stage_at<LAYOUT>(ll, j) : -1;
+    R.hpos2 = item ? split_stage_at<LAYOUT>(ll, w + nm + j) : -1;
+}
+
+// The windowed pair values at positions t + 512 m, combined into a and b.
+template <int LAYOUT>
+__device__ __forceinline__ void split_gather_finish(const TileDesc &td, int t, const SplitGather &R, float *stage,
+                                                    float *win, const float2 *tw8, float2 (&va)[8], float2 (&vb)[8]) {
+    const int w = td.wlen, nm = td.n_main, vel = td.vel;
+    __builtin_assume(w > 0 && w <= 32);
+    const float c_s1 = td.c_s1, c_m1 = td.c_m1, c_s2 = td.c_s2, c_m2 = td.c_m2;
+    const bool ok1 = LAYOUT == 0 || 1 < td.count;
+    float raw[16][2];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int pos = t + m * kSplitT;
+        const bool mid = pos >= w && pos < w + nm;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            raw[m][q] = q == 0 || ok1 ? R.ra[m][q] + R.rb[m][q] : 0.f;
+            if (mid) stage[split_stage_at<LAYOUT>(q, pos)] = raw[m][q];
+        }
+    }
+    if (R.hpos1 >= 0) {
+        if constexpr (LAYOUT == 1) {
+            *reinterpret_cast<float2 *>(&stage[R.hpos1]) =
+                make_float2(R.h1a[0] + R.h1b[0], ok1 ? R.h1a[1] + R.h1b[1] : 0.f);
+            *reinterpret_cast<float2 *>(&stage[R.hpos2]) =
+                make_float2(R.h2a[0] + R.h2b[0], ok1 ? R.h2a[1] + R.h2b[1] : 0.f);
+        } else {
+            stage[R.hpos1] = R.h1a[0] + R.h1b[0];
+            stage[R.hpos2] = R.h2a[0] + R.h2b[0];
+        }
+    }
+    if ((int)threadIdx.x <= 2 * w) win[threadIdx.x] = R.win;
+    __syncthreads();
+    // positions n and n + 4096 in turn, combined at once (fewer live registers)
+    auto windowed = [&](int m) {
+        const int i = t + m * kSplitT;
+        const bool left = i < w, right = i >= w + nm && i < 2 * w + nm, mid = i >= w && i < w + nm;
+        const int j = i - w - nm;
+        float out[2] = {0.f, 0.f};
+        if (left || right) {
+            const int mpos = left ? (vel ? w - i : w - 1 - i) : (vel ? nm - 2 - j : nm - 1 - j);
+            const float cs = left ? c_s1 : c_s2, cm = left ? c_m1 : c_m2;
+            const float wv = win[left ? i : w + 1 + j];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const float sv = stage[split_stage_at<LAYOUT>(q, i)];
+                const float refl = stage[split_stage_at<LAYOUT>(q, w + mpos)];
+                out[q] = __builtin_fmaf(sv, cs, refl * cm) * wv;
+            }
+        }
+        return make_float2(mid ? raw[m][0] : out[0], mid ? raw[m][1] : out[1]);
+    };
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const float2 z0 = windowed(m), z1 = windowed(m + 8);
+        va[m] = cadd(z0, z1);
+        vb[m] = cmul(csub(z0, z1), tw8[t + m * kSplitT]);
+    }
+}
+
+// Update of the outputs at positions base + t + 512 m (out = t_n - s coef L;
+// the last stage also t_n = out * PML), as finish_updates.
+template <int LAYOUT>
+__device__ __forceinline__ float split_update(const StageArgs &a, const TileDesc &td, int t, int base,
+                                              const float2 (&y)[8]) {
+    const int w = td.wlen, n_out = td.n_out;
+    const bool ok1 = 1 < td.count;
+    const int pstep = td.o_ps * 4;
+    const int obase = (td.o_f + td.line0 * td.o_ls) * 4, lstep = td.o_ls * 4;
+    const bool use_pml = a.last && td.is_pml;
+    const Rsrc rt = block_rsrc(a.tn, td.m_dom, td.m_size);
+    const Rsrc rp = block_rsrc(a.pml, td.m_dom, td.m_size);
+    const Rsrc ro = block_rsrc(a.out, td.m_dom, td.m_size);
+    const float sc = a.s * td.coef;
+    int off[8];
+    float tnv[8][2], pm[8][2];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int pos = base + t + m * kSplitT - w;
+        off[m] = pos >= 0 && pos < n_out ? obase + pos * pstep : kOOB;
+        if constexpr (LAYOUT == 1) {
+            const float2 x = bload2(rt, off[m]);
+            const float2 f = use_pml ? bload2(rp, off[m]) : make_float2(1.f, 1.f);
+            tnv[m][0] = x.x; tnv[m][1] = x.y; pm[m][0] = f.x; pm[m][1] = f.y;
+        } else {
+            const int o1 = off[m] == kOOB || !ok1 ? kOOB : off[m] + lstep;
+            tnv[m][0] = bload(rt, off[m]);
+            tnv[m][1] = bload(rt, o1);
+            pm[m][0] = use_pml ? bload(rp, off[m]) : 1.f;
+            pm[m][1] = use_pml ? bload(rp, o1) : 1.f;
+        }
+    }
+    float mx = 0.f;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const float n0 = __builtin_fmaf(-sc, y[m].x, tnv[m][0]);
+        const float n1 = __builtin_fmaf(-sc, y[m].y, tnv[m][1]);
+        if (off[m] != kOOB) mx = fmaxf(mx, ok1 ? fmaxf(fabsf(n0), fabsf(n1)) : fabsf(n0));
+        if (LAYOUT == 1 && ok1) {
+            bstore2(ro, off[m], n0, n1);
+            if (a.last) bstore2(rt, off[m], n0 * pm[m][0], n1 * pm[m][1]);
+        } else {
+            const int o1 = off[m] == kOOB || !ok1 ? kOOB : off[m] + lstep;
+            bstore(ro, off[m], n0);
+            bstore(ro, o1, n1);
+            if (a.last) {
+                bstore(rt, off[m], n0 * pm[m][0]);
+                bstore(rt, o1, n1 * pm[m][1]);
+            }
+        }
+    }
+    return mx;
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ void split_tile(const StageArgs &a, const TileDesc &td, float2 *lds) {
+    const int t = threadIdx.x;
+    float2 *buf0 = lds, *buf1 = lds + LdsMap<4096, 1, LAYOUT>::SIZE;
+    __shared__ float win[2 * 32 + 1];
+    SplitGather R;
+    split_gather_issue<LAYOUT>(a, td, t, R);
+    const float2 *tw8 = a.ctab + td.tw_off, *tw4 = a.ctab + td.tw2_off, *h = a.ctab + td.h_off;
+    float2 va[8], vb[8];
+    split_gather_finish<LAYOUT>(td, t, R, reinterpret_cast<float *>(lds), win, tw8, va, vb);
+    const float2 w0[kTwPass] = {};
+    // even bins: e = IFFT(H[2k] FFT(a)); the staging is read before the first pass writes
+    __syncthreads();
+    fft_run_ahead<4096, 1, LAYOUT, false>(va, 0, t, buf0, buf1, tw4, w0);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) va[m] = cmul(va[m], h[2 * (t + m * kSplitT)]);
+    __syncthreads();
+    fft_run_ahead<4096, 1, LAYOUT, true>(va, 0, t, buf0, buf1, tw4, w0);
+    // odd bins: o = IFFT(H[2k+1] FFT(b))
+    __syncthreads();
+    fft_run_ahead<4096, 1, LAYOUT, false>(vb, 0, t, buf0, buf1, tw4, w0);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) vb[m] = cmul(vb[m], h[2 * (t + m * kSplitT) + 1]);
+    __syncthreads();
+    fft_run_ahead<4096, 1, LAYOUT, true>(vb, 0, t, buf0, buf1, tw4, w0);
+    // y[n] = e[n] + W^-n o[n], y[n + 4096] = e[n] - W^-n o[n]
+    float2 lo[8], hi[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        float2 wn = tw8[t + m * kSplitT];
+        wn.y = -wn.y;
+        const float2 wo = cmul(vb[m], wn);
+        lo[m] = cadd(va[m], wo);
+        hi[m] = csub(va[m], wo);
+    }
+    const float mx = fmaxf(split_update<LAYOUT>(a, td, t, 0, lo), split_update<LAYOUT>(a, td, t, 4096, hi));
+    track_max(a, td.scene, td.vel ? 0 : 1, mx);
+}
+
 // One extra workgroup of a launch samples the receivers (p0 = px0 + py0 of the
 // previous step, read from the arena `p0a`/`p0b` point at in the first stage).
 __device__ void sample_receivers(const StageArgs &a) {
@@ -664,6 +911,14 @@ __global__ void __launch_bounds__(F / 8) stage_big_kernel(StageArgs a) {
     extern __shared__ float2 lds_dyn[];
     const TileDesc td = a.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
     derivative_tile<F / 8, F, F, FL>(a, td, lds_dyn, lds_dyn + LdsMap<F, 1, 0>::SIZE);
+}
+
+// two blocks per CU: 16 waves, at most 128 VGPRs each
+__global__ void __launch_bounds__(kSplitT) __attribute__((amdgpu_waves_per_eu(4))) stage_split_kernel(StageArgs a) {
+    extern __shared__ float2 lds_split[];
+    const TileDesc td = a.tiles[tile_index(blockIdx.x, gridDim.x, a.remap)];
+    if (td.dir == 0) split_tile<0>(a, td, lds_split);
+    else split_tile<1>(a, td, lds_split);
 }
 
 __global__ void receiver_kernel(const float *__restrict__ arena, const ReceiverCell *__restrict__ cells, int nrec,
@@ -1536,11 +1791,9 @@ int launch_stage(pstd_ctx *ctx, int k, int sample_slot) {
         else hipLaunchKernelGGL((stage_big_kernel<4096, kEarly>), dim3(n4), dim3(512), lds, fs, a);
     });
     if (n8) timed_launch(ctx, fs, kClass8192, [&] {
+        // one line pair per 512-thread block as two 4096-point transforms
         a.tiles = ctx->d_tiles + ns + n4;
-        size_t lds = 2 * sizeof(float2) * LdsMap<8192, 1, 0>::SIZE;
-        if (fl == kLate) hipLaunchKernelGGL((stage_big_kernel<8192, kLate>), dim3(n8), dim3(1024), lds, fs, a);
-        else if (fl == kMid) hipLaunchKernelGGL((stage_big_kernel<8192, kMid>), dim3(n8), dim3(1024), lds, fs, a);
-        else hipLaunchKernelGGL((stage_big_kernel<8192, kEarly>), dim3(n8), dim3(1024), lds, fs, a);
+        hipLaunchKernelGGL(stage_split_kernel, dim3(n8), dim3(kSplitT), kSplitLds, fs, a);
     });
     if (fork) HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
     if (ng) timed_launch(ctx, ctx->stream, kClassGemm, [&] {
@@ -1635,16 +1888,12 @@ int pstd_ctx_create(int device, pstd_ctx **out) {
     if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess) ctx->ev_fork = nullptr;
     if (hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) ctx->ev_join = nullptr;
     // the long-line classes need more than 64 KiB of LDS per workgroup
-    const size_t lds8 = 2 * sizeof(float2) * LdsMap<8192, 1, 0>::SIZE, lds4 = 2 * sizeof(float2) * LdsMap<4096, 1, 0>::SIZE;
+    const size_t lds4 = 2 * sizeof(float2) * LdsMap<4096, 1, 0>::SIZE;
     const hipFuncAttribute attr = hipFuncAttributeMaxDynamicSharedMemorySize;
-    const void *big8[3] = {(const void *)stage_big_kernel<8192, kEarly>, (const void *)stage_big_kernel<8192, kLate>,
-                           (const void *)stage_big_kernel<8192, kMid>};
     const void *big4[3] = {(const void *)stage_big_kernel<4096, kEarly>, (const void *)stage_big_kernel<4096, kLate>,
                            (const void *)stage_big_kernel<4096, kMid>};
-    bool lds_ok = true;
-    for (int f = 0; f < 3; ++f)
-        lds_ok = lds_ok && hipFuncSetAttribute(big8[f], attr, lds8) == hipSuccess &&
-                 hipFuncSetAttribute(big4[f], attr, lds4) == hipSuccess;
+    bool lds_ok = hipFuncSetAttribute((const void *)stage_split_kernel, attr, kSplitLds) == hipSuccess;
+    for (int f = 0; f < 3; ++f) lds_ok = lds_ok && hipFuncSetAttribute(big4[f], attr, lds4) == hipSuccess;
     lds_ok = lds_ok &&
              hipFuncSetAttribute((const void *)stage_gemm_kernel<_Float16>, attr, kGemmLdsBytes) == hipSuccess &&
              hipFuncSetAttribute((const void *)stage_gemm_kernel<__bf16>, attr, kGemmLdsBytes) == hipSuccess;
@@ -2182,6 +2431,18 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan, const std::vec
             ctab.insert(ctab.end(), tw.begin(), tw.end());
         }
         g.tw_off = tt->second;
+        g.tw2_off = 0;
+        if (g.F == 8192) {
+            // the split transform's two 4096-point halves
+            auto t2 = tw_cache.find(4096);
+            if (t2 == tw_cache.end()) {
+                std::vector<float2> tw;
+                build_twiddles(4096, tw);
+                t2 = tw_cache.emplace(4096, (int)ctab.size()).first;
+                ctab.insert(ctab.end(), tw.begin(), tw.end());
+            }
+            g.tw2_off = t2->second;
+        }
         auto wt = win_cache.find(s.wlen);
         if (wt == win_cache.end()) {
             std::vector<float> w;

@@ -177,12 +177,13 @@ def test_rigid_box_standing_wave(mode):
     assert np.abs(got - expect).max() < 5e-6
 
 
-@pytest.mark.parametrize("n", [2000, 4100])
-def test_long_lines(n):
-    """F = 4096 and F = 8192 lines (the long-line kernels)."""
+@pytest.mark.parametrize("n, lines", [(2000, 60), (4100, 60), (4100, 61)])
+def test_long_lines(n, lines):
+    """F = 4096 and F = 8192 lines (the long-line kernels; 8192-point lines as
+    two 4096-point halves), with an odd line count: a last tile of one line."""
     conf = P.PSTDConfiguration()
     W.base_settings(conf, 4)
-    conf.Domains = [W.domain(0, 0, n, 60, 1.0)]
+    conf.Domains = [W.domain(0, 0, n, lines, 1.0)]
     conf.Speakers = [(W._world_pos(n / 2), W._world_pos(30.0), 0.0)]
     conf.Receivers = [(W._world_pos(n / 2 + 7.25), W._world_pos(33.25), 0.0)]
     k, dev = device(conf)
@@ -192,12 +193,13 @@ def test_long_lines(n):
     compare_fields(k, dev, o, 5e-6)
 
 
-@pytest.mark.parametrize("n", [2000, 4100])
-def test_long_columns(n):
-    """F = 4096 and F = 8192 columns (the long-line kernels, Y layout)."""
+@pytest.mark.parametrize("n, lines", [(2000, 60), (4100, 60), (4100, 61)])
+def test_long_columns(n, lines):
+    """F = 4096 and F = 8192 columns (the long-line kernels, Y layout), with an
+    odd column count."""
     conf = P.PSTDConfiguration()
     W.base_settings(conf, 4)
-    conf.Domains = [W.domain(0, 0, 60, n, 1.0)]
+    conf.Domains = [W.domain(0, 0, lines, n, 1.0)]
     conf.Speakers = [(W._world_pos(30.0), W._world_pos(n / 2), 0.0)]
     conf.Receivers = [(W._world_pos(33.25), W._world_pos(n / 2 + 7.25), 0.0)]
     k, dev = device(conf)
