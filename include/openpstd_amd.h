@@ -166,8 +166,8 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan);
 /* Derivative form (pstd_ctx_set_form, before the plan is uploaded).
  * PSTD_FORM_FFT: every line derivative as openPSTD computes it (windowed line,
  *   FFT, spectral factor, inverse FFT) in FP32.
- * PSTD_FORM_GEMM_F16: segments whose lines and outputs fill 128-line x
- *   256-output tiles at least to 3/4 apply the segment's dense derivative
+ * PSTD_FORM_GEMM_F16: segments whose lines and outputs fill blocks of 128
+ *   lines x 256 outputs at least to 3/4 apply the segment's dense derivative
  *   matrix (pstd_derivative_matrix) on MFMA with FP16 hi + lo split operands
  *   (22-bit significands; the inputs scaled per scene by a power of two, from
  *   the largest value the previous RK stage wrote) and FP32 accumulation; the
