@@ -1299,10 +1299,17 @@ __device__ __forceinline__ void gemm_stage(E *ulds, int buf, const UChunk &u, fl
             const E h = (E)x;
             hi[j] = h;
             lo[j] = (E)(x - (float)h);
-            if (dot) gv[q] = __builtin_fmaf(raw, u.dk[q][j], gv[q]);
         }
         *reinterpret_cast<V8 *>(U0 + row * kGP + kq) = hi;
         *reinterpret_cast<V8 *>(U1 + row * kGP + kq) = lo;
+    }
+    // the dot row (a uniform branch: only the last output tile of a line block)
+    if (dot) {
+#pragma unroll
+        for (int q = 0; q < kGSlots; ++q)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                gv[q] = __builtin_fmaf(HASB ? u.a[q][j] + u.b[q][j] : u.a[q][j], u.dk[q][j], gv[q]);
     }
 }
 
