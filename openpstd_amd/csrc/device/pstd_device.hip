@@ -33,6 +33,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../../include/openpstd_amd.h"
@@ -1410,8 +1411,25 @@ __device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt
     while (c + 1 < min(cm0, nch)) step(generic);
     {
         const MainU m = main_u<DIR, HASB>(r, td, line_ok);
-        auto mainl = [&](int k0) { gemm_load_u_main<DIR, HASB>(m, k0, dotr, dot, u); };
-        while (c + 1 < cm1) step(mainl);
+        // the same step with the LDS buffer parity known at compile time
+        // (LDS addresses as immediate offsets): two chunks per iteration
+        auto step_p = [&](auto parity) {
+            constexpr int buf = decltype(parity)::value;
+            gemm_load_u_main<DIR, HASB>(m, (c + 1) * kGK, dotr, dot, u);
+            gemm_dma_d(dr, dlds + (buf ^ 1) * kGemmDChunk, c + 1);
+            if (active) gemm_chunk<E, DIR>(ulds, dlds, buf, lg, og, acc);
+            gemm_stage<E, DIR, HASB>(ulds, buf ^ 1, u, sc, dot, gv);
+            __syncthreads();
+            ++c;
+        };
+        using P0 = std::integral_constant<int, 0>;
+        using P1 = std::integral_constant<int, 1>;
+        if ((c & 1) && c + 1 < cm1) step_p(P1{});
+        while (c + 2 < cm1) {
+            step_p(P0{});
+            step_p(P1{});
+        }
+        if (c + 1 < cm1) step_p(P0{});
     }
     while (c + 1 < nch) step(generic);
     if (active) gemm_chunk<E, DIR>(ulds, dlds, c & 1, lg, og, acc);
