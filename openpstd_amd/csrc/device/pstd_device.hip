@@ -1453,28 +1453,65 @@ __device__ __forceinline__ void gemm_tile(const StageArgs &a, const GemmTile &gt
             return ok ? (td.o_f + (td.line0 + line) * td.o_ls + n * td.o_ps) * 4 : kOOB;
         };
         float tnv[2][2][16];
+        if (64 * lg + 64 <= td.count && 64 * og + 64 <= gt.n_g - gt.n0) {
+            // the wave's whole 64 x 64 block lies inside the tile: a per-lane
+            // offset and, per value, a uniform one (the buffer instruction's
+            // scalar offset; no address arithmetic on the vector unit)
+            const int base = (td.o_f + (td.line0 + lrow0) * td.o_ls + nrow0 * td.o_ps) * 4;
+            const int sl = td.o_ls * 4, sp = td.o_ps * 4;
+            auto uoff = [&](int i, int j, int q) {
+                const int rowd = 32 * i + 8 * (q >> 2) + (q & 3), cold = 32 * j;
+                return DIR == 0 ? rowd * sl + cold * sp : cold * sl + rowd * sp;
+            };
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    bool ok;
-                    tnv[i][j][q] = bload(rt, offset(i, j, q, ok));
-                }
+                    for (int q = 0; q < 16; ++q)
+                        tnv[i][j][q] = __builtin_bit_cast(
+                            float, __builtin_amdgcn_raw_buffer_load_b32(rt, base, uoff(i, j, q), 0));
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    bool ok;
-                    const int off = offset(i, j, q, ok);
-                    const float nv = __builtin_fmaf(-sc_upd, acc[i][j][q] * inv, tnv[i][j][q]);
-                    bstore(ro, off, nv);
-                    if (a.last) bstore(rt, off, nv * (use_pml ? bload(rp, off) : 1.f));
-                    if (ok) mx = fmaxf(mx, fabsf(nv));
-                }
+                    for (int q = 0; q < 16; ++q) {
+                        const int so = uoff(i, j, q);
+                        const float nv = __builtin_fmaf(-sc_upd, acc[i][j][q] * inv, tnv[i][j][q]);
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, nv), ro, base, so, 0);
+                        if (a.last) {
+                            const float pm = use_pml ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                                                  rp, base, so, 0))
+                                                     : 1.f;
+                            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, nv * pm), rt, base, so, 0);
+                        }
+                        mx = fmaxf(mx, fabsf(nv));
+                    }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        bool ok;
+                        tnv[i][j][q] = bload(rt, offset(i, j, q, ok));
+                    }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        bool ok;
+                        const int off = offset(i, j, q, ok);
+                        const float nv = __builtin_fmaf(-sc_upd, acc[i][j][q] * inv, tnv[i][j][q]);
+                        bstore(ro, off, nv);
+                        if (a.last) bstore(rt, off, nv * (use_pml ? bload(rp, off) : 1.f));
+                        if (ok) mx = fmaxf(mx, fabsf(nv));
+                    }
+        }
     }
     if (dot) {
         // output n_out - 1 of every line: the FP32 dot products (no scaling)
