@@ -83,6 +83,18 @@ typedef struct pstd_domain_conf {
  * assertions aborts on every rel.x >= ny). */
 #define PSTD_RECEIVER_PHYSICAL 0
 #define PSTD_RECEIVER_REFERENCE 1
+/* Spectral interpolation at the receiver's exact location
+ * (Receiver::compute_with_si, kernel/core/Receiver.cpp:85-121, which openPSTD
+ * keeps switched off, :48): the pressure of the receiver's domain shifted
+ * along X by the fractional part of the receiver's x (every row of the domain
+ * and the window rows of its Y neighbours, each line with its own neighbours
+ * and stencil as Domain::calc resolves them), then the receiver's column
+ * shifted along Y by the fractional part of y, read at the receiver's cell.
+ * Fractional parts are location - (int)location (openPSTD's constructor
+ * stores the integer cell there, :41); wavenumbers are those of each line's
+ * FFT length.  PSTDSettings::spectral_interpolation alone does not select it,
+ * as in openPSTD. */
+#define PSTD_RECEIVER_SPECTRAL 2
 
 /* PSTDConfiguration (kernel/KernelInterface.h:214-241). */
 typedef struct pstd_config {
@@ -135,11 +147,16 @@ typedef struct pstd_plan_segment {
     int32_t wlen;                   /* window half length w (Domain::calc halves it for short domains) */
     int32_t table_len;              /* length of the wavenumber table derfact is read from */
     float c_side1, c_main1, c_side2, c_main2; /* get_rho_array stencil coefficients */
+    int32_t interp_only;            /* 1: not part of the time step; the X pressure lines of a domain
+                                       the RK loop does not update along X (rigid, or a PML strip
+                                       that skips X), present for spectral receivers beside it */
 } pstd_plan_segment;
 
 typedef struct pstd_plan_receiver {
     int32_t domain;
     int32_t index;                  /* row-major index into the domain's p0 */
+    int32_t spectral;               /* 1: spectral interpolation (PSTD_RECEIVER_SPECTRAL) */
+    float offset_x, offset_y;       /* fractional offsets from the cell, in cells, in [0, 1) */
 } pstd_plan_receiver;
 
 typedef struct pstd_plan_desc {
@@ -348,6 +365,15 @@ int pstd_batch_prepare(pstd_kernel *const *kernels, int n, int device, pstd_ctx 
  * the element count (out may be NULL to query it) or a negative status. */
 long long pstd_derivative_matrix(int F, int table_len, int field, double dx, int wlen, int n_main, double patch_error,
                                  const double coef[4], double *out, size_t cap);
+/* One row of the spectral receivers' shift operator: the K = n_main + 2w
+ * weights of [side1 tail | main | side2 head] whose dot product with a
+ * pressure line is spatderp3 (kernel/core/kernel_functions.cpp:123-318) of
+ * that line with the shift factors of Receiver::get_fft_factors
+ * (kernel/core/Receiver.cpp:57-73, offset in cells) in place of derfact,
+ * read at output `row` (< n_main + 1).  Returns K (out may be NULL to query
+ * it) or a negative status. */
+long long pstd_shift_operator_row(int F, int table_len, double dx, int wlen, int n_main, double patch_error,
+                                  const double coef[4], double offset, int row, double *out, size_t cap);
 /* Plan the kernel hands to the device (segments etc.) — for tests. */
 int pstd_kernel_segment_count(pstd_kernel *k);
 int pstd_kernel_get_segment(pstd_kernel *k, int i, pstd_plan_segment *out);

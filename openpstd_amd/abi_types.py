@@ -16,6 +16,7 @@ FIELD_IDS = {"p0": 0, "px0": 1, "py0": 2, "vx0": 3, "vy0": 4}
 
 PSTD_RECEIVER_PHYSICAL = 0
 PSTD_RECEIVER_REFERENCE = 1
+PSTD_RECEIVER_SPECTRAL = 2
 
 PSTD_FORM_FFT = 0
 PSTD_FORM_GEMM_F16 = 1
@@ -88,6 +89,7 @@ class pstd_plan_segment(C.Structure):
         ("c_main1", C.c_float),
         ("c_side2", C.c_float),
         ("c_main2", C.c_float),
+        ("interp_only", C.c_int32),
     ]
 
 

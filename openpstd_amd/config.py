@@ -166,8 +166,9 @@ class DomainConf:
 class PSTDConfiguration:
     """kernel/KernelInterface.h:214-241.
 
-    ``receiver_mode`` selects how the nearest-cell receiver indexes p0
-    (abi_types.PSTD_RECEIVER_PHYSICAL or PSTD_RECEIVER_REFERENCE).
+    ``receiver_mode`` selects how receivers read p0: nearest cell
+    (abi_types.PSTD_RECEIVER_PHYSICAL or PSTD_RECEIVER_REFERENCE) or spectral
+    interpolation at the exact location (PSTD_RECEIVER_SPECTRAL).
     """
 
     def __init__(self):

@@ -98,7 +98,7 @@ struct TileDesc {
 };
 
 struct ReceiverCell {
-    long long px, py;  // arena element offsets of the cell's px0 and py0
+    long long px, py;  // arena element offsets of the cell's px0 and py0; px < 0: spectral receiver
 };
 
 struct StageArgs {
@@ -838,6 +838,7 @@ __device__ void sample_receivers(const StageArgs &a) {
     if (a.slot < 0) return;
     for (int r = threadIdx.x; r < a.nrec; r += blockDim.x) {
         const ReceiverCell c = a.cells[r];
+        if (c.px < 0) continue;  // spectral receiver
         a.samples[(size_t)a.slot * a.nrec + r] = a.p0a[c.px] + a.p0b[c.py];
     }
 }
@@ -927,8 +928,73 @@ __global__ void receiver_kernel(const float *__restrict__ arena, const ReceiverC
     // p0 = px0 + py0 (kernel/Solver.cpp:114-116)
     for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
         const ReceiverCell c = cells[r];
+        if (c.px < 0) continue;  // spectral receiver: sampled by the interpolation passes
         samples[(size_t)slot * nrec + r] = arena[c.px] + arena[c.py];
     }
+}
+
+// Spectral-interpolation receivers (Receiver::compute_with_si,
+// kernel/core/Receiver.cpp:85-121), two passes after each step.  The
+// fractional shift of one pressure line, read at one output, is a fixed linear
+// function of the line's K = n + 2w raw values [side-1 tail | main | side-2
+// head] (p0 = px + py): the row of the line operator (shift filter composed
+// with the window and reflections) built in double on the host.  Pass 1: one
+// wave per line of the domains the receiver's Y line reads, the dot product
+// of its values with its X-shift row, into the receiver's column buffer
+// [side-1 tail | domain rows | side-2 head].  Pass 2: one wave per receiver,
+// the column buffer against the Y-shift row of the receiver's row.
+struct SILine {
+    long long m_px, m_py;    // arena offsets of the line's first main value in px0, py0
+    long long s1_px, s1_py;  // first of the w side-1 tail values; -1: no neighbour (zeros)
+    long long s2_px, s2_py;  // first of the w side-2 head values; -1: zeros
+    int n_main, w;
+    int coef;                // first of the line's K weights in the weight table
+    int dst;                 // element of the column buffer
+};
+
+struct SIPoint {
+    int coef, K;             // Y-shift weights of the receiver's row
+    int col;                 // first element of the receiver's column buffer
+    int rec;                 // receiver index (sample position)
+};
+
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void si_line_kernel(const float *__restrict__ arena, const SILine *__restrict__ lines,
+                                                      int nlines, const float *__restrict__ coef,
+                                                      float *__restrict__ col) {
+    const int lane = threadIdx.x & 63, j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= nlines) return;
+    const SILine L = lines[j];
+    const int w = L.w, n = L.n_main, K = n + 2 * w;
+    const float *cf = coef + L.coef;
+    float acc = 0.f;
+    for (int k = lane; k < K; k += 64) {
+        float u = 0.f;
+        if (k < w) {
+            if (L.s1_px >= 0) u = arena[L.s1_px + k] + arena[L.s1_py + k];
+        } else if (k < w + n) {
+            u = arena[L.m_px + (k - w)] + arena[L.m_py + (k - w)];
+        } else if (L.s2_px >= 0) {
+            u = arena[L.s2_px + (k - w - n)] + arena[L.s2_py + (k - w - n)];
+        }
+        acc = fmaf(cf[k], u, acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) col[L.dst] = acc;
+}
+
+__global__ __launch_bounds__(64) void si_point_kernel(const float *__restrict__ col, const SIPoint *__restrict__ pts,
+                                                      const float *__restrict__ coef, float *__restrict__ samples,
+                                                      int nrec, int slot) {
+    const SIPoint P = pts[blockIdx.x];
+    float acc = 0.f;
+    for (int k = threadIdx.x; k < P.K; k += 64) acc = fmaf(coef[P.coef + k], col[P.col + k], acc);
+    acc = wave_sum(acc);
+    if (threadIdx.x == 0) samples[(size_t)slot * nrec + P.rec] = acc;
 }
 
 struct P0Range {
@@ -1661,6 +1727,12 @@ struct pstd_ctx {
     unsigned *umax = nullptr;     // per-scene maxima of the written values, 3 slots x scenes x 2 classes
     MaxRange *d_max_ranges = nullptr;
     ReceiverCell *d_cells = nullptr;
+    // spectral receivers: line jobs, per-receiver jobs, weights, column buffer
+    SILine *d_si_lines = nullptr;
+    SIPoint *d_si_points = nullptr;
+    float *d_si_coef = nullptr;
+    float *d_si_col = nullptr;
+    int n_si_lines = 0, n_si = 0;
     P0Range *d_ranges = nullptr;
     int n_ranges = 0;
     float *d_samples = nullptr;
@@ -1718,6 +1790,14 @@ struct pstd_ctx {
         umax = nullptr;
         d_max_ranges = nullptr;
         (void)hipFree(d_cells);
+        (void)hipFree(d_si_lines);
+        (void)hipFree(d_si_points);
+        (void)hipFree(d_si_coef);
+        (void)hipFree(d_si_col);
+        d_si_lines = nullptr;
+        d_si_points = nullptr;
+        d_si_coef = d_si_col = nullptr;
+        n_si_lines = n_si = 0;
         (void)hipFree(d_ranges);
         (void)hipFree(d_samples);
         arena = pml = ftab = d_samples = nullptr;
@@ -1996,6 +2076,14 @@ static int enqueue_steps(pstd_ctx *ctx, int chunk, int base = 0) {
             int rc = launch_stage(ctx, k, s == 0 ? -1 : base + s - 1);
             if (rc) return rc;
             if (k == 0) ctx->p0_explicit = false;
+        }
+        if (ctx->n_si > 0) {
+            // spectral receivers of this step, from its raw stage-5 output
+            hipLaunchKernelGGL(si_line_kernel, dim3((ctx->n_si_lines + 3) / 4), dim3(256), 0, ctx->stream, ctx->S(1),
+                               ctx->d_si_lines, ctx->n_si_lines, ctx->d_si_coef, ctx->d_si_col);
+            hipLaunchKernelGGL(si_point_kernel, dim3(ctx->n_si), dim3(64), 0, ctx->stream, ctx->d_si_col,
+                               ctx->d_si_points, ctx->d_si_coef, ctx->d_samples, ctx->nrec, base + s);
+            HIP_TRY(hipGetLastError());
         }
     }
     if (ctx->nrec > 0) {
@@ -2300,6 +2388,147 @@ static int add_gemm_tiles(pstd_ctx *ctx, const pstd_plan_desc *plan, const pstd_
 }
 
 // dom_scene: scene of every domain (a batch of scenes), or nullptr (one scene).
+// Line jobs and weights of the spectral receivers (Receiver::compute_with_si,
+// kernel/core/Receiver.cpp:85-121): for each, the Y pressure segment of its
+// domain through its column gives the Y line (neighbours, stencil, window);
+// every row of the domain and the w window rows of each Y neighbour are
+// shifted along X with the X pressure segment holding that row; the weights
+// are rows of the segments' line operators with the shift filter
+// (plan_tables.h), rounded once to float.
+static int build_spectral_receivers(pstd_ctx *ctx, const pstd_plan_desc *plan) {
+    std::vector<SILine> lines;
+    std::vector<SIPoint> points;
+    std::vector<float> coef;
+    int col_total = 0;
+    std::map<std::pair<int, int>, int> row_cache;  // (segment, output column) -> weights, per receiver
+    auto find_segment = [&](int dom, int dir, int line) -> int {
+        for (int k = 0; k < plan->n_segments; ++k) {
+            const pstd_plan_segment &s = plan->segments[k];
+            if (s.domain == dom && s.direction == dir && s.field == 0 && s.line_begin <= line && line < s.line_end)
+                return k;
+        }
+        return -1;
+    };
+    auto shape_of = [&](const pstd_plan_segment &s) {
+        const DomainDev &d = ctx->doms[s.domain];
+        const int n = s.direction == 0 ? d.nx : d.ny;
+        return pstdamd::SegmentShape{next_pow2(n + 2 * s.wlen), s.table_len, 0, (double)plan->dx, s.wlen, n,
+                                     (double)plan->patch_error, (double)s.c_side1, (double)s.c_main1,
+                                     (double)s.c_side2, (double)s.c_main2};
+    };
+    auto add_weights = [&](const pstd_plan_segment &s, double offset, int out) {
+        const pstdamd::SegmentShape sh = shape_of(s);
+        const std::vector<double> row = pstdamd::operator_row(sh, pstdamd::shift_filter(sh.F, sh.table_len, offset), out);
+        const int at = (int)coef.size();
+        for (double v : row) coef.push_back((float)v);
+        return at;
+    };
+    for (int r = 0; r < plan->n_receivers; ++r) {
+        const pstd_plan_receiver &rc = plan->receivers[r];
+        if (!rc.spectral) continue;
+        if (!(rc.offset_x >= 0.f && rc.offset_x < 1.f && rc.offset_y >= 0.f && rc.offset_y < 1.f)) {
+            ctx->err = "spectral receiver " + std::to_string(r) + ": offsets outside [0, 1)";
+            return PSTD_ERR_INVALID;
+        }
+        const DomainDev &c = ctx->doms[rc.domain];
+        const int cx = rc.index % c.nx, cy = rc.index / c.nx;
+        const int ky = find_segment(rc.domain, 1, cx);
+        if (ky < 0) {
+            ctx->err = "spectral receiver " + std::to_string(r) + ": no Y pressure segment through its column";
+            return PSTD_ERR_INVALID;
+        }
+        const pstd_plan_segment &sy = plan->segments[ky];
+        const int w = sy.wlen, col0 = col_total;
+        row_cache.clear();
+        // the rows to shift along X: (domain, row, output column, column buffer element)
+        struct Row {
+            int dom, row, out, dst;
+        };
+        std::vector<Row> rows;
+        for (int y = 0; y < c.ny; ++y) rows.push_back({rc.domain, y, cx, col0 + w + y});
+        if (sy.side1 >= 0) {
+            const DomainDev &n1 = ctx->doms[sy.side1];
+            for (int j = 0; j < w; ++j) rows.push_back({sy.side1, n1.ny - w + j, cx + sy.side1_offset, col0 + j});
+        }
+        if (sy.side2 >= 0)
+            for (int j = 0; j < w; ++j) rows.push_back({sy.side2, j, cx + sy.side2_offset, col0 + w + c.ny + j});
+        for (const Row &q : rows) {
+            const int kx = find_segment(q.dom, 0, q.row);
+            if (kx < 0) {
+                ctx->err = "spectral receiver " + std::to_string(r) + ": domain " + std::to_string(q.dom) +
+                           " has no X pressure segment through row " + std::to_string(q.row);
+                return PSTD_ERR_INVALID;
+            }
+            const pstd_plan_segment &sx = plan->segments[kx];
+            const DomainDev &d = ctx->doms[q.dom];
+            if (q.out < 0 || q.out >= d.nx) {
+                ctx->err = "spectral receiver " + std::to_string(r) + ": column outside a neighbour";
+                return PSTD_ERR_INVALID;
+            }
+            auto it = row_cache.find({kx, q.out});
+            if (it == row_cache.end()) {
+                int at;
+                try {
+                    at = add_weights(sx, (double)rc.offset_x, q.out);
+                } catch (const std::exception &e) {
+                    ctx->err = std::string("spectral receiver: ") + e.what();
+                    return PSTD_ERR_INVALID;
+                }
+                it = row_cache.emplace(std::make_pair(kx, q.out), at).first;
+            }
+            SILine L{};
+            L.m_px = d.base + d.f_px + (long long)q.row * d.nx;
+            L.m_py = d.base + d.f_py + (long long)q.row * d.nx;
+            L.s1_px = L.s1_py = L.s2_px = L.s2_py = -1;
+            if (sx.side1 >= 0) {
+                const DomainDev &n = ctx->doms[sx.side1];
+                const long long rr = (long long)(q.row + sx.side1_offset) * n.nx + n.nx - sx.wlen;
+                L.s1_px = n.base + n.f_px + rr;
+                L.s1_py = n.base + n.f_py + rr;
+            }
+            if (sx.side2 >= 0) {
+                const DomainDev &n = ctx->doms[sx.side2];
+                const long long rr = (long long)(q.row + sx.side2_offset) * n.nx;
+                L.s2_px = n.base + n.f_px + rr;
+                L.s2_py = n.base + n.f_py + rr;
+            }
+            L.n_main = d.nx;
+            L.w = sx.wlen;
+            L.coef = it->second;
+            L.dst = q.dst;
+            lines.push_back(L);
+        }
+        SIPoint P{};
+        try {
+            P.coef = add_weights(sy, (double)rc.offset_y, cy);
+        } catch (const std::exception &e) {
+            ctx->err = std::string("spectral receiver: ") + e.what();
+            return PSTD_ERR_INVALID;
+        }
+        P.K = c.ny + 2 * w;
+        P.col = col0;
+        P.rec = r;
+        points.push_back(P);
+        col_total += P.K;
+    }
+    if (points.empty()) return PSTD_OK;
+    HIP_TRY(hipMalloc(&ctx->d_si_lines, lines.size() * sizeof(SILine)));
+    HIP_TRY(hipMalloc(&ctx->d_si_points, points.size() * sizeof(SIPoint)));
+    HIP_TRY(hipMalloc(&ctx->d_si_coef, coef.size() * sizeof(float)));
+    HIP_TRY(hipMalloc(&ctx->d_si_col, (size_t)col_total * sizeof(float)));
+    HIP_TRY(hipMemcpyAsync(ctx->d_si_lines, lines.data(), lines.size() * sizeof(SILine), hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->d_si_points, points.data(), points.size() * sizeof(SIPoint), hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->d_si_coef, coef.data(), coef.size() * sizeof(float), hipMemcpyHostToDevice,
+                           ctx->stream));
+    // window rows of missing neighbours are never written: zeros
+    HIP_TRY(hipMemsetAsync(ctx->d_si_col, 0, (size_t)col_total * sizeof(float), ctx->stream));
+    ctx->n_si_lines = (int)lines.size();
+    ctx->n_si = (int)points.size();
+    return PSTD_OK;
+}
+
 static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan, const std::vector<int> *dom_scene = nullptr) {
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -2465,6 +2694,13 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan, const std::vec
                 g.s2_size = dn.size;
             }
         }
+        if (s.interp_only) {
+            if (s.direction != 0 || s.field != 0) {
+                ctx->err = "interpolation-only segment " + std::to_string(si) + " is not an X pressure segment";
+                return PSTD_ERR_INVALID;
+            }
+            continue;  // read by the spectral receivers only
+        }
         // output field: X pressure -> vx, X velocity -> px, Y pressure -> vy, Y velocity -> py
         if (s.direction == 0) {
             g.o_f = s.field == 0 ? dm.f_vx : dm.f_px;
@@ -2586,9 +2822,14 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan, const std::vec
             return PSTD_ERR_INVALID;
         }
         const DomainDev &d = ctx->doms[rc.domain];
-        cells.push_back({d.base + d.f_px + rc.index, d.base + d.f_py + rc.index});
+        if (rc.spectral) cells.push_back({-1, -1});
+        else cells.push_back({d.base + d.f_px + rc.index, d.base + d.f_py + rc.index});
     }
     ctx->nrec = (int)cells.size();
+    {
+        int rc = build_spectral_receivers(ctx, plan);
+        if (rc) return rc;
+    }
     std::vector<P0Range> ranges;
     for (const DomainDev &d : ctx->doms) ranges.push_back({d.base + d.f_px, d.base + d.f_py, (long long)d.nx * d.ny});
     ctx->n_ranges = (int)ranges.size();

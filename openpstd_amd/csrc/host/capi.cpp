@@ -278,6 +278,23 @@ long long pstd_derivative_matrix(int F, int table_len, int field, double dx, int
     }
 }
 
+long long pstd_shift_operator_row(int F, int table_len, double dx, int wlen, int n_main, double patch_error,
+                                  const double coef[4], double offset, int row, double *out, size_t cap) {
+    if (!coef) return PSTD_ERR_INVALID;
+    pstdamd::SegmentShape s{F, table_len, 0, dx, wlen, n_main, patch_error, coef[0], coef[1], coef[2], coef[3]};
+    if (!s.valid() || row < 0 || row >= s.n_out()) return PSTD_ERR_INVALID;
+    try {
+        const size_t n = (size_t)s.K();
+        if (!out) return (long long)n;
+        if (cap < n) return PSTD_ERR_INVALID;
+        std::vector<double> r = pstdamd::operator_row(s, pstdamd::shift_filter(F, table_len, offset), row);
+        std::memcpy(out, r.data(), n * sizeof(double));
+        return (long long)n;
+    } catch (const std::exception &) {
+        return PSTD_ERR_INVALID;
+    }
+}
+
 int pstd_kernel_segment_count(pstd_kernel *k) {
     if (!k || !k->k->configured()) return -1;
     return (int)k->k->get_scene().segments().size();

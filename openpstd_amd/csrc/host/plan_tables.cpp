@@ -24,6 +24,17 @@ std::vector<std::complex<double>> derivative_filter(int F, int L, int field, dou
     return out;
 }
 
+std::vector<std::complex<double>> shift_filter(int F, int L, double delta) {
+    std::vector<std::complex<double>> out(F, 0.0);
+    for (int m = 0; m <= F / 2; ++m) {
+        std::complex<double> h = std::polar(1.0, M_PI * m * delta / (L / 2)) / (double)F;
+        if (m == F / 2) h = h.real();
+        out[m] = h;
+        if (m > 0 && m < F / 2) out[F - m] = std::conj(h);
+    }
+    return out;
+}
+
 std::vector<double> window_coefficients(int w, double patch_error) {
     std::vector<double> out(2 * w + 1);
     const double alpha = (patch_error - 40) / 20.0 + 1;
@@ -59,18 +70,15 @@ void idft_pow2(std::vector<std::complex<double>> &a) {
 
 }  // namespace
 
-std::vector<double> derivative_matrix(const SegmentShape &s) {
-    const int F = s.F, w = s.w, nm = s.n_main, no = s.n_out(), K = s.K(), vel = s.field;
-    if (!s.valid()) throw std::invalid_argument("derivative_matrix: inconsistent segment shape");
-    // c = circular convolution kernel of the filter (1/F already in H)
-    std::vector<std::complex<double>> h = derivative_filter(F, s.table_len, s.field, s.dx);
-    idft_pow2(h);
-    std::vector<double> c(F);
-    for (int r = 0; r < F; ++r) c[r] = h[r].real();
+// Rows [r0, r1) of the line operator whose filter has the circular
+// convolution kernel c: D[i][k] = sum over the window/reflection terms (q, a)
+// of input k of c[(w + i - q) mod F] * a.
+static std::vector<double> line_operator(const SegmentShape &s, const std::vector<double> &c, int r0, int r1) {
+    const int F = s.F, w = s.w, nm = s.n_main, K = s.K(), vel = s.field, nr = r1 - r0;
     const std::vector<double> win = window_coefficients(w, s.patch_error);
-    std::vector<double> D((size_t)no * K, 0.0);
+    std::vector<double> D((size_t)nr * K, 0.0);
     auto add = [&](int k, int q, double a) {
-        for (int i = 0; i < no; ++i) D[(size_t)i * K + k] += c[((w + i - q) % F + F) % F] * a;
+        for (int i = 0; i < nr; ++i) D[(size_t)i * K + k] += c[((w + r0 + i - q) % F + F) % F] * a;
     };
     for (int k = 0; k < K; ++k) {
         if (k < w) {
@@ -88,6 +96,25 @@ std::vector<double> derivative_matrix(const SegmentShape &s) {
         }
     }
     return D;
+}
+
+static std::vector<double> convolution_kernel(std::vector<std::complex<double>> h) {
+    idft_pow2(h);
+    std::vector<double> c(h.size());
+    for (size_t r = 0; r < h.size(); ++r) c[r] = h[r].real();
+    return c;
+}
+
+std::vector<double> derivative_matrix(const SegmentShape &s) {
+    if (!s.valid()) throw std::invalid_argument("derivative_matrix: inconsistent segment shape");
+    // c = circular convolution kernel of the filter (1/F already in H)
+    return line_operator(s, convolution_kernel(derivative_filter(s.F, s.table_len, s.field, s.dx)), 0, s.n_out());
+}
+
+std::vector<double> operator_row(const SegmentShape &s, const std::vector<std::complex<double>> &H, int row) {
+    if (!s.valid() || (int)H.size() != s.F || row < 0 || row >= s.n_out())
+        throw std::invalid_argument("operator_row: inconsistent segment shape, filter or row");
+    return line_operator(s, convolution_kernel(H), row, row + 1);
 }
 
 }  // namespace pstdamd

@@ -18,6 +18,14 @@ namespace pstdamd {
 // in.  field: 0 pressure, 1 velocity.
 std::vector<std::complex<double>> derivative_filter(int F, int L, int field, double dx);
 
+// Fractional-shift filter of a line for spectral interpolation: the factors
+// Receiver::get_fft_factors forms (kernel/core/Receiver.cpp:57-73),
+// exp(i k_m delta dx) with k_m as in derivative_filter (so exp(i pi m delta /
+// (L/2))) on bins 0..F/2, bin F/2 real, Hermitian-extended, 1/F folded in.
+// Applied to a pressure line in place of derfact, output i is the
+// band-limited line at position i + delta (cells).
+std::vector<std::complex<double>> shift_filter(int F, int L, double delta);
+
 // get_window_coefficients (kernel/core/kernel_functions.cpp:327-333): 2w+1 values.
 std::vector<double> window_coefficients(int w, double patch_error);
 
@@ -51,5 +59,10 @@ struct SegmentShape {
 // w+1 .. 2 from the end (velocity), heads from its first w (pressure) or
 // values 1..w (velocity); the caller supplies them in that order.
 std::vector<double> derivative_matrix(const SegmentShape &s);
+
+// Row `row` (< n_out) of the same operator with the filter H (F bins, 1/F
+// folded in) in place of the derivative filter: K = n_main + 2w weights of
+// [side1 tail | main | side2 head].
+std::vector<double> operator_row(const SegmentShape &s, const std::vector<std::complex<double>> &H, int row);
 
 }  // namespace pstdamd

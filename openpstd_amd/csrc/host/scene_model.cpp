@@ -381,6 +381,8 @@ void SceneModel::add_speaker(float x, float y) {
 // Scene::add_receiver (kernel/core/Scene.cpp:254-265), Receiver::compute_with_nn
 // (kernel/core/Receiver.cpp:80-83).
 void SceneModel::add_receiver(float x, float y, int mode) {
+    if (mode != PSTD_RECEIVER_PHYSICAL && mode != PSTD_RECEIVER_REFERENCE && mode != PSTD_RECEIVER_SPECTRAL)
+        throw PSTDSolverError("unknown receiver mode " + std::to_string(mode));
     int container = -1;
     for (int li = 0; li < (int)list_.size(); ++li) {
         const DomainModel &d = pool_[list_[li]];
@@ -393,6 +395,11 @@ void SceneModel::add_receiver(float x, float y, int mode) {
     r.domain = container;
     r.cell_x = (int)x;
     r.cell_y = (int)y;
+    // Receiver::compute_with_si shifts by location - grid cell (the intended
+    // grid_offset; kernel/core/Receiver.cpp:41 stores the cell itself there)
+    r.spectral = mode == PSTD_RECEIVER_SPECTRAL;
+    r.off_x = x - (float)r.cell_x;
+    r.off_y = y - (float)r.cell_y;
     int relx = r.cell_x - d.tlx, rely = r.cell_y - d.tly;
     int row = rely, col = relx;
     if (mode == PSTD_RECEIVER_REFERENCE) {
@@ -419,90 +426,118 @@ void SceneModel::build_segments() {
         if (d.rigid()) continue;
         for (int cd = 0; cd < 2; ++cd) {
             if (!d.update[cd]) continue;
-            for (int ct = 0; ct < 2; ++ct) {
-                const bool X = cd == 0;
-                Interval own = X ? Interval{d.tly, d.bry()} : Interval{d.tlx, d.brx()};
-                std::vector<char> left(own.b - own.a, 1);  // lines not yet assigned
-                const std::vector<int> &n1 = X ? d.nb[kLeft] : d.nb[kBottom];
-                const std::vector<int> &n2 = X ? d.nb[kRight] : d.nb[kTop];
-                // openPSTD's Domain::calc sets d1 = shared_from_this() in the
-                // first segment without a side-1 neighbour and never resets it
-                // inside the j loop (kernel/core/Domain.cpp:148-150,210-233), so
-                // a second such segment would read the domain's own field as its
-                // side 1 with rho_self.  Scenes reaching that are rejected.
-                int no_side1_segments = 0;
-                for (size_t i = 0; i <= n1.size(); ++i) {
-                    int d1 = i < n1.size() ? n1[i] : -1;
-                    for (size_t j = 0; j <= n2.size(); ++j) {
-                        int d2 = j < n2.size() ? n2[j] : -1;
-                        auto range = [&](int pi) {
-                            const DomainModel &o = pool_[pi];
-                            return X ? Interval{o.tly, o.bry()} : Interval{o.tlx, o.brx()};
-                        };
-                        Interval lim = own;
-                        if (d1 >= 0) lim = intersect(lim, range(d1));
-                        if (d2 >= 0) lim = intersect(lim, range(d2));
-                        int lo = 0, hi = 0, count = 0;
-                        for (int v = std::max(lim.a, own.a); v < lim.b; ++v)
-                            if (left[v - own.a]) {
-                                if (count == 0) lo = v;
-                                hi = v;
-                                ++count;
-                            }
-                        if (count == 0) continue;
-                        if (hi - lo + 1 != count)
-                            throw PSTDSolverError("non-contiguous segment in Domain::calc is not supported (domain " +
-                                                  std::to_string(d.id) + ", lines " + std::to_string(lo) + ".." +
-                                                  std::to_string(hi) + ", " + std::to_string(count) + " free)");
-                        for (int v = lo; v <= hi; ++v) left[v - own.a] = 0;
-                        if (d1 < 0 && ++no_side1_segments > 1)
-                            throw PSTDSolverError(
-                                "two line segments without a side-1 neighbour in one Domain::calc pass (domain " +
-                                std::to_string(d.id) + ", " + (X ? "X" : "Y") +
-                                "): openPSTD reuses the domain itself as side 1 there; not supported");
-                        int primary = X ? d.nx : d.ny;
-                        int wlen = window_;
-                        while (wlen > primary) wlen /= 2;
-                        int n_total = 2 * wlen + primary + (ct == 0 ? 1 : 0);
-                        int table_len = next_pow2(n_total);
-                        int main_len = ct == 0 ? primary : primary + 1;
-                        int F = next_pow2(main_len + 2 * wlen);
-                        if (table_len < F)
-                            throw PSTDSolverError("domain size with n+2w a power of two: openPSTD reads its "
-                                                  "wavenumber table past the end");
-                        if (table_len != F)
-                            throw PSTDSolverError("domain size with n+2w a power of two: openPSTD's wavenumber "
-                                                  "table and FFT length disagree");
-                        pstd_plan_segment s{};
-                        s.domain = li;
-                        s.direction = cd;
-                        s.field = ct;
-                        s.line_begin = lo - own.a;
-                        s.line_end = hi + 1 - own.a;
-                        s.wlen = wlen;
-                        s.table_len = table_len;
-                        double r1 = d1 >= 0 ? pool_[d1].rho : d.rho;
-                        double r2 = d2 >= 0 ? pool_[d2].rho : d.rho;
-                        float c[4];
-                        stencil(r1, d.rho, r2, ct, c);
-                        s.c_side1 = c[0];
-                        s.c_main1 = c[1];
-                        s.c_side2 = c[2];
-                        s.c_main2 = c[3];
-                        s.side1 = d1 >= 0 ? list_index_of_pool(d1) : -1;
-                        s.side2 = d2 >= 0 ? list_index_of_pool(d2) : -1;
-                        s.side1_offset = d1 >= 0 ? (X ? d.tly - pool_[d1].tly : d.tlx - pool_[d1].tlx) : 0;
-                        s.side2_offset = d2 >= 0 ? (X ? d.tly - pool_[d2].tly : d.tlx - pool_[d2].tlx) : 0;
-                        for (int dd : {d1, d2}) {
-                            if (dd < 0) continue;
-                            int len = (X ? pool_[dd].nx : pool_[dd].ny) + (ct == 1 ? 1 : 0);
-                            if (len < wlen + (ct == 1 ? 1 : 0))
-                                throw PSTDSolverError("neighbour narrower than the window");
-                        }
-                        segments_.push_back(s);
-                    }
+            for (int ct = 0; ct < 2; ++ct) line_segments(li, cd, ct, 0);
+        }
+    }
+    // spectral receivers read the X-shifted pressure of the neighbours of
+    // their Y line (Receiver::calc_domain_fields, kernel/core/Receiver.cpp:
+    // 117-121); a neighbour the step does not update along X gets its X
+    // pressure lines as interpolation-only segments
+    for (const ReceiverModel &r : receivers_) {
+        if (!r.spectral) continue;
+        const DomainModel &c = pool_[list_[r.domain]];
+        const int col = r.cell_x - c.tlx;
+        const pstd_plan_segment *sy = nullptr;
+        for (const pstd_plan_segment &s : segments_)
+            if (s.domain == r.domain && s.direction == 1 && s.field == 0 && s.line_begin <= col && col < s.line_end)
+                sy = &s;
+        if (!sy) throw PSTDSolverError("spectral receiver: its domain has no Y pressure line through it");
+        const int sides[2] = {sy->side1, sy->side2};
+        for (int nb : sides) {
+            if (nb < 0) continue;
+            bool has_x = false;
+            for (const pstd_plan_segment &s : segments_)
+                if (s.domain == nb && s.direction == 0 && s.field == 0) has_x = true;
+            if (!has_x) line_segments(nb, 0, 0, 1);
+        }
+    }
+}
+
+// The segments of Domain::calc for (domain list index li, direction cd,
+// field ct).
+void SceneModel::line_segments(int li, int cd, int ct, int interp_only) {
+    const DomainModel &d = pool_[list_[li]];
+    const bool X = cd == 0;
+    Interval own = X ? Interval{d.tly, d.bry()} : Interval{d.tlx, d.brx()};
+    std::vector<char> left(own.b - own.a, 1);  // lines not yet assigned
+    const std::vector<int> &n1 = X ? d.nb[kLeft] : d.nb[kBottom];
+    const std::vector<int> &n2 = X ? d.nb[kRight] : d.nb[kTop];
+    // openPSTD's Domain::calc sets d1 = shared_from_this() in the
+    // first segment without a side-1 neighbour and never resets it
+    // inside the j loop (kernel/core/Domain.cpp:148-150,210-233), so
+    // a second such segment would read the domain's own field as its
+    // side 1 with rho_self.  Scenes reaching that are rejected.
+    int no_side1_segments = 0;
+    for (size_t i = 0; i <= n1.size(); ++i) {
+        int d1 = i < n1.size() ? n1[i] : -1;
+        for (size_t j = 0; j <= n2.size(); ++j) {
+            int d2 = j < n2.size() ? n2[j] : -1;
+            auto range = [&](int pi) {
+                const DomainModel &o = pool_[pi];
+                return X ? Interval{o.tly, o.bry()} : Interval{o.tlx, o.brx()};
+            };
+            Interval lim = own;
+            if (d1 >= 0) lim = intersect(lim, range(d1));
+            if (d2 >= 0) lim = intersect(lim, range(d2));
+            int lo = 0, hi = 0, count = 0;
+            for (int v = std::max(lim.a, own.a); v < lim.b; ++v)
+                if (left[v - own.a]) {
+                    if (count == 0) lo = v;
+                    hi = v;
+                    ++count;
                 }
+            if (count == 0) continue;
+            if (hi - lo + 1 != count)
+                throw PSTDSolverError("non-contiguous segment in Domain::calc is not supported (domain " +
+                                      std::to_string(d.id) + ", lines " + std::to_string(lo) + ".." +
+                                      std::to_string(hi) + ", " + std::to_string(count) + " free)");
+            for (int v = lo; v <= hi; ++v) left[v - own.a] = 0;
+            if (d1 < 0 && ++no_side1_segments > 1)
+                throw PSTDSolverError(
+                    "two line segments without a side-1 neighbour in one Domain::calc pass (domain " +
+                    std::to_string(d.id) + ", " + (X ? "X" : "Y") +
+                    "): openPSTD reuses the domain itself as side 1 there; not supported");
+            int primary = X ? d.nx : d.ny;
+            int wlen = window_;
+            while (wlen > primary) wlen /= 2;
+            int n_total = 2 * wlen + primary + (ct == 0 ? 1 : 0);
+            int table_len = next_pow2(n_total);
+            int main_len = ct == 0 ? primary : primary + 1;
+            int F = next_pow2(main_len + 2 * wlen);
+            if (table_len < F)
+                throw PSTDSolverError("domain size with n+2w a power of two: openPSTD reads its "
+                                      "wavenumber table past the end");
+            if (table_len != F)
+                throw PSTDSolverError("domain size with n+2w a power of two: openPSTD's wavenumber "
+                                      "table and FFT length disagree");
+            pstd_plan_segment s{};
+            s.domain = li;
+            s.direction = cd;
+            s.field = ct;
+            s.line_begin = lo - own.a;
+            s.line_end = hi + 1 - own.a;
+            s.wlen = wlen;
+            s.table_len = table_len;
+            double r1 = d1 >= 0 ? pool_[d1].rho : d.rho;
+            double r2 = d2 >= 0 ? pool_[d2].rho : d.rho;
+            float c[4];
+            stencil(r1, d.rho, r2, ct, c);
+            s.c_side1 = c[0];
+            s.c_main1 = c[1];
+            s.c_side2 = c[2];
+            s.c_main2 = c[3];
+            s.interp_only = interp_only;
+            s.side1 = d1 >= 0 ? list_index_of_pool(d1) : -1;
+            s.side2 = d2 >= 0 ? list_index_of_pool(d2) : -1;
+            s.side1_offset = d1 >= 0 ? (X ? d.tly - pool_[d1].tly : d.tlx - pool_[d1].tlx) : 0;
+            s.side2_offset = d2 >= 0 ? (X ? d.tly - pool_[d2].tly : d.tlx - pool_[d2].tlx) : 0;
+            for (int dd : {d1, d2}) {
+                if (dd < 0) continue;
+                int len = (X ? pool_[dd].nx : pool_[dd].ny) + (ct == 1 ? 1 : 0);
+                if (len < wlen + (ct == 1 ? 1 : 0))
+                    throw PSTDSolverError("neighbour narrower than the window");
             }
+            segments_.push_back(s);
         }
     }
 }
@@ -596,7 +631,15 @@ void SceneModel::plan(pstd_plan_desc *desc, std::vector<pstd_plan_domain> &doms,
         doms.push_back(p);
     }
     rcv.clear();
-    for (const ReceiverModel &r : receivers_) rcv.push_back({r.domain, r.index});
+    for (const ReceiverModel &r : receivers_) {
+        pstd_plan_receiver p{};
+        p.domain = r.domain;
+        p.index = r.index;
+        p.spectral = r.spectral ? 1 : 0;
+        p.offset_x = r.off_x;
+        p.offset_y = r.off_y;
+        rcv.push_back(p);
+    }
     *desc = pstd_plan_desc{};
     desc->n_domains = (int)doms.size();
     desc->domains = doms.data();

@@ -51,6 +51,8 @@ struct ReceiverModel {
     int domain = -1;  // list index
     int index = -1;   // row-major p0 index
     int cell_x = 0, cell_y = 0;
+    bool spectral = false;         // PSTD_RECEIVER_SPECTRAL
+    float off_x = 0, off_y = 0;    // location - cell (cells)
 };
 
 class SceneModel {
@@ -85,6 +87,7 @@ private:
     void add_speaker(float x, float y);
     void add_receiver(float x, float y, int mode);
     void build_segments();
+    void line_segments(int li, int cd, int ct, int interp_only);
 
     pstd_settings settings_{};
     float dt_ = 0.f;

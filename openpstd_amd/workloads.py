@@ -14,6 +14,7 @@ generated: openPSTD's wavenumber table and FFT length disagree for them.
 """
 import numpy as np
 
+from . import abi_types as T
 from .config import PSTDConfiguration, DomainConf, RK_COEFFICIENTS, _f32
 
 DX = _f32(0.2)
@@ -98,6 +99,17 @@ def lshape(seed=0, steps=2000, absorptions=(0.0, 0.5, 1.0)):
         bx, by, bw, bh = boxes[rng.randint(3)]
         rx, ry = bx + rng.randint(1, bw - 1), by + rng.randint(1, bh - 1)
         conf.Receivers.append((_world_pos(rx + 0.25), _world_pos(ry + 0.25), 0.0))
+    return conf
+
+
+def lshape_spectral(steps=400):
+    """The L-shape of seed 0 with spectral-interpolation receivers: its four
+    seeded receivers (cell + 0.25) and six more at other offsets, beside the
+    A|B (x = 256) and A|C (y = 128) interfaces and an outer edge."""
+    conf = lshape(0, steps)
+    for x, y in [(254.6, 60.3), (257.35, 70.8), (100.45, 126.9), (140.8, 129.15), (3.3, 40.7), (128.0, 64.5)]:
+        conf.Receivers.append((_world_pos(x), _world_pos(y), 0.0))
+    conf.receiver_mode = T.PSTD_RECEIVER_SPECTRAL
     return conf
 
 

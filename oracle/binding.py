@@ -56,6 +56,8 @@ def lib():
         L.oracle_read_pml.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
         L.oracle_neighbours.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, C.c_int]
         L.oracle_receiver_cell.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
+        L.oracle_receiver_value.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.oracle_shift_factors.argtypes = [C.c_double, C.c_int, C.c_int, C.c_double, _dp]
         L.oracle_step.argtypes = [C.c_void_p, C.c_int, _dp, C.c_void_p, C.c_void_p]
         L.oracle_rk_stage.argtypes = [C.c_void_p, C.c_int]
         L.oracle_push_values.argtypes = [C.c_void_p]
@@ -95,6 +97,14 @@ def window(w, pe, precision=32):
     out = np.zeros(2 * w + 1)
     lib().oracle_window(w, pe, precision, _ptr(out))
     return out
+
+
+def shift_factors(dx, n, offset, precision=64):
+    """Receiver::get_fft_factors over the discretization of n (spectral interpolation)."""
+    L = lib().oracle_discretization(dx, n, precision, None, None, None, None)
+    out = np.zeros(2 * L)
+    lib().oracle_shift_factors(dx, n, precision, offset, _ptr(out))
+    return out[0::2] + 1j * out[1::2]
 
 
 def discretization(dx, n, precision=32):
@@ -207,6 +217,13 @@ class OracleScene:
         d, idx = C.c_int32(), C.c_int32()
         lib().oracle_receiver_cell(self._h, r, C.byref(d), C.byref(idx))
         return d.value, idx.value
+
+    def receiver_value(self, r):
+        """Receiver r's value on the current state, without stepping."""
+        out = np.zeros(1)
+        if lib().oracle_receiver_value(self._h, r, _ptr(out)) != 0:
+            raise RuntimeError("oracle: " + lib().oracle_error().decode())
+        return float(out[0])
 
     def step(self, nsteps, frames=None):
         """Advances nsteps; returns (nsteps, n_receivers) samples.  When ``frames`` is a

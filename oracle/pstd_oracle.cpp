@@ -47,7 +47,10 @@ static thread_local std::string g_error;
 // 0 = the restatement as written; 1 = side-1 halo read one cell off;
 // 2 = reflection coefficients of get_rho_array negated; 3 = PML attenuation
 // profiles reversed; 4 = Y lines take the top neighbour as side 1 (the
-// geometric order) instead of the bottom one.  Set only through
+// geometric order) instead of the bottom one; 5 = spectral interpolation
+// shifts by minus the offsets; 6 = spectral interpolation swaps the X and Y
+// offsets; 7 = spectral interpolation's Y pass reads the row after the
+// receiver's.  Set only through
 // oracle_set_mutation, by tests, to show that the invariants fail on them.
 static int g_mutation = 0;
 // Threads over the lines of one spatderp3 call (1: serial); set only by fixture
@@ -555,7 +558,18 @@ struct Domain {
 
     void create_attenuation_array(CalcDir cd, bool ascending, Mat<R> &pp, Mat<R> &pv);
     void compute_pml_matrices();
+    // One line segment of Domain::calc: neighbours (nullptr: none) and the
+    // global line range [rs, re).
+    struct Segment {
+        Domain *d1, *d2;
+        int rs, re;
+    };
+    std::vector<Segment> segments(CalcDir cd);
     void calc(CalcDir cd, CalcType ct);
+    // Domain::calc with `factors(disc)` in place of derfact, into `dest`
+    // (same shape as the derivative it would compute).
+    template <class Factors>
+    void calc_into(CalcDir cd, CalcType ct, Factors factors, Mat<R> &dest);
 };
 
 template <class R>
@@ -567,6 +581,7 @@ template <class R>
 struct Receiver {
     float loc[3];
     Point grid;
+    float off[2] = {0, 0};  // location - grid cell: the offsets spectral interpolation shifts by
     Domain<R> *container = nullptr;
     int id = 0;
     std::vector<R> received;
@@ -811,6 +826,8 @@ struct Scene {
         Receiver<R> r;
         std::memcpy(r.loc, loc, sizeof loc);
         r.grid = Point((int)x, (int)y, (int)z);
+        r.off[0] = x - (float)r.grid.x;
+        r.off[1] = y - (float)r.grid.y;
         r.container = container;
         r.id = id;
         receivers.push_back(r);
@@ -836,6 +853,8 @@ struct Scene {
 
     void compute_rk_step(int rk_step);
     void compute_timestep(int frame, double *samples, oracle_frame_cb cb, void *user);
+    R spectral_value(const Receiver<R> &r);
+    R receiver_value(const Receiver<R> &r);
 };
 
 // Domain::create_attenuation_array (Domain.cpp:743-780)
@@ -904,15 +923,15 @@ void Domain<R>::compute_pml_matrices() {
     }
 }
 
-// Domain::calc (Domain.cpp:95-391)
+// Domain::calc's neighbour-pair loop (Domain.cpp:95-233): the line segments
+// of one direction and the neighbours each reads.
 template <class R>
-void Domain<R>::calc(CalcDir cd, CalcType ct) {
-    Scene<R> &sc = *scene;
+std::vector<typename Domain<R>::Segment> Domain<R>::segments(CalcDir cd) {
     std::vector<int> own = get_range(cd);
     const bool geometric_y = g_mutation == 4;
     std::vector<Domain *> &d1s = cd == X ? left : (geometric_y ? top : bottom);
     std::vector<Domain *> &d2s = cd == X ? right : (geometric_y ? bottom : top);
-    Mat<R> &source = ct == PRESSURE ? (cd == X ? L.Lpx : L.Lpy) : (cd == X ? L.Lvx : L.Lvy);
+    std::vector<Segment> out;
     // Domain.cpp:148-150,210-233: the first segment without a side-1
     // neighbour sets d1 = shared_from_this(), which stays set for the rest of
     // the j loop; a second such segment would read the domain itself as side 1.
@@ -945,6 +964,29 @@ void Domain<R>::calc(CalcDir cd, CalcType ct) {
             }
             int rs = *std::min_element(inter.begin(), inter.end());
             int re = *std::max_element(inter.begin(), inter.end()) + 1;
+            out.push_back({d1, d2, rs, re});
+        }
+    }
+    return out;
+}
+
+// Domain::calc (Domain.cpp:95-391)
+template <class R>
+void Domain<R>::calc(CalcDir cd, CalcType ct) {
+    Mat<R> &source = ct == PRESSURE ? (cd == X ? L.Lpx : L.Lpy) : (cd == X ? L.Lvx : L.Lvy);
+    calc_into(cd, ct, [ct](const Discretization<R> &disc) -> const std::vector<std::complex<R>> & {
+        return ct == PRESSURE ? disc.pressure : disc.velocity;
+    }, source);
+}
+
+template <class R>
+template <class Factors>
+void Domain<R>::calc_into(CalcDir cd, CalcType ct, Factors factors, Mat<R> &source) {
+    Scene<R> &sc = *scene;
+    for (const Segment &sg : segments(cd)) {
+        Domain *d1 = sg.d1, *d2 = sg.d2;
+        const int rs = sg.rs, re = sg.re;
+        {
             int full_range = re - rs;
             int primary = cd == X ? size.x : size.y;
             int result_dim = primary;
@@ -975,7 +1017,7 @@ void Domain<R>::calc(CalcDir cd, CalcType ct) {
             if (!z1) side1 = ct == PRESSURE ? s1->current.p0 : (cd == X ? s1->current.vx0 : s1->current.vy0);
             if (!z2) side2 = ct == PRESSURE ? s2->current.p0 : (cd == X ? s2->current.vx0 : s2->current.vy0);
             const Discretization<R> &disc = sc.get_discretization(N_total);
-            const std::vector<std::complex<R>> &derfact = ct == PRESSURE ? disc.pressure : disc.velocity;
+            const std::vector<std::complex<R>> &derfact = factors(disc);
             RhoArray<R> rho_array = get_rho_array<R>(s1->rho, rho, s2->rho);
             Mat<R> res;
             if (cd == X) {
@@ -997,6 +1039,17 @@ void Domain<R>::calc(CalcDir cd, CalcType ct) {
             }
         }
     }
+}
+
+// Receiver::get_fft_factors (Receiver.cpp:57-73): exp(offset * dx * k * c)
+// over a line's discretization (c = +i on bins 0..F/2, -i above).  The table
+// is the line's own (openPSTD sizes it from 2 * wave_length + n + 1, which
+// gives other wavenumbers whenever wave_length > 0).
+template <class R>
+static std::vector<std::complex<R>> shift_factors(const Discretization<R> &disc, R dx, R offset) {
+    std::vector<std::complex<R>> f(disc.wave_numbers.size());
+    for (size_t i = 0; i < f.size(); ++i) f[i] = std::exp(offset * dx * disc.wave_numbers[i] * disc.complex_factors[i]);
+    return f;
 }
 
 // SingleThreadSolver / MultiThreadSolver::compute_rk_step (Solver.cpp:95-151)
@@ -1073,12 +1126,63 @@ void Scene<R>::compute_timestep(int fr, double *samples, oracle_frame_cb cb, voi
             for (size_t k = 0; k < d->current.vy0.a.size(); ++k) d->current.vy0.a[k] *= d->pml.vy.a[k];
         }
     for (size_t r = 0; r < receivers.size(); ++r) {
-        int row, col;
-        if (!receiver_cell(receivers[r], &row, &col)) throw std::runtime_error("receiver cell outside its domain");
-        R v = receivers[r].container->current.p0(row, col);
+        R v = receiver_value(receivers[r]);
         receivers[r].received.push_back(v);
         if (samples) samples[r] = (double)v;
     }
+}
+
+// Receiver::compute_local_pressure (Receiver.cpp:46-55): nearest cell, or
+// spectral interpolation in PSTD_RECEIVER_SPECTRAL mode.
+template <class R>
+R Scene<R>::receiver_value(const Receiver<R> &r) {
+    if (receiver_mode == PSTD_RECEIVER_SPECTRAL) return spectral_value(r);
+    int row, col;
+    if (!receiver_cell(r, &row, &col)) throw std::runtime_error("receiver cell outside its domain");
+    return r.container->current.p0(row, col);
+}
+
+// Receiver::compute_with_si (Receiver.cpp:85-121): every row of the domains
+// the receiver's Y line reads is shifted along X by off[0]
+// (calc_domain_fields: Domain::calc with the shift factors), the receiver's
+// column of the result is shifted along Y by off[1] with that line's
+// neighbours, stencil and window (spatderp3), and the receiver's row is read.
+// The Y line's neighbours are the ones Domain::calc resolves for that column
+// (openPSTD's compute_with_si takes the TOP and BOTTOM neighbours at the
+// location and pairs BOTTOM with the rho of TOP, which dereferences a missing
+// neighbour; the two agree where both exist with one density).
+template <class R>
+R Scene<R>::spectral_value(const Receiver<R> &r) {
+    Domain<R> *c = r.container;
+    const R dx = (R)settings.s.grid_spacing;
+    const R sign = g_mutation == 5 ? (R)-1 : (R)1;
+    const R ox = sign * (R)r.off[g_mutation == 6 ? 1 : 0], oy = sign * (R)r.off[g_mutation == 6 ? 0 : 1];
+    const int gx = r.grid.x, rel_x = gx - c->top_left.x, rel_y = r.grid.y - c->top_left.y;
+    if (rel_x < 0 || rel_x >= c->size.x || rel_y < 0 || rel_y >= c->size.y)
+        throw std::runtime_error("receiver cell outside its domain");
+    const typename Domain<R>::Segment *seg = nullptr;
+    std::vector<typename Domain<R>::Segment> segs = c->segments(Y);
+    for (const auto &sg : segs)
+        if (sg.rs <= gx && gx < sg.re) seg = &sg;
+    if (!seg) throw std::runtime_error("spectral receiver: no Y line through the receiver");
+    // X-shifted pressure of a domain, column `col` (Receiver::calc_domain_fields)
+    auto shifted_column = [&](Domain<R> *d, int col) {
+        Mat<R> m(d->size.y, d->size.x + 1);
+        d->calc_into(X, PRESSURE, [&](const Discretization<R> &disc) { return shift_factors<R>(disc, dx, ox); }, m);
+        return m.block(0, col, d->size.y, 1);
+    };
+    Mat<R> p2 = shifted_column(c, rel_x);
+    Domain<R> *s1 = seg->d1 ? seg->d1 : c, *s2 = seg->d2 ? seg->d2 : c;
+    Mat<R> p1 = seg->d1 ? shifted_column(seg->d1, gx - seg->d1->top_left.x) : Mat<R>(c->size.y, 1);
+    Mat<R> p3 = seg->d2 ? shifted_column(seg->d2, gx - seg->d2->top_left.x) : Mat<R>(c->size.y, 1);
+    int wlen = settings.window_size();
+    while (wlen > c->size.y) wlen /= 2;
+    const std::vector<R> wind = get_window_coefficients<R>(wlen, (int)settings.s.patcherror);
+    const Discretization<R> &disc = get_discretization(2 * wlen + c->size.y + 1);
+    const std::vector<std::complex<R>> fy = shift_factors<R>(disc, dx, oy);
+    const RhoArray<R> rho_array = get_rho_array<R>(s1->rho, c->rho, s2->rho);
+    Mat<R> res = spatderp3<R>(p1, p2, p3, fy, rho_array, wind, wlen, PRESSURE, Y, fftc);
+    return res(rel_y + (g_mutation == 7 ? 1 : 0), 0);
 }
 
 template <class R>
@@ -1205,6 +1309,20 @@ static int disc_impl(double dx, int n, double *wn, double *cf, double *pf, doubl
         if (vf) { vf[2 * i] = d.velocity[i].real(); vf[2 * i + 1] = d.velocity[i].imag(); }
     }
     return L;
+}
+
+template <class R>
+static int shift_impl(double dx, int n, double offset, double *out) {
+    auto d = discretize_wave_numbers<R>((R)dx, match_number(n));
+    auto f = shift_factors<R>(d, (R)dx, (R)offset);
+    for (size_t i = 0; i < f.size(); ++i) {
+        out[2 * i] = f[i].real();
+        out[2 * i + 1] = f[i].imag();
+    }
+    return (int)f.size();
+}
+int oracle_shift_factors(double dx, int n, int precision, double offset, double *out) {
+    return precision == 32 ? shift_impl<float>(dx, n, offset, out) : shift_impl<double>(dx, n, offset, out);
 }
 
 int oracle_discretization(double dx, int n, int precision, double *wn, double *cf, double *pf, double *vf) {
@@ -1415,6 +1533,22 @@ int oracle_receiver_cell(const oracle_scene *o, int r, int32_t *domain, int32_t 
         }
     });
     return ok;
+}
+int oracle_receiver_value(oracle_scene *o, int r, double *out) {
+    try {
+        int ok = -1;
+        DISPATCH(o, {
+            if (r >= 0 && r < (int)S.receivers.size()) {
+                *out = (double)S.receiver_value(S.receivers[r]);
+                ok = 0;
+            }
+        });
+        if (ok) g_last_error = "receiver index out of range";
+        return ok;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
 }
 int oracle_step(oracle_scene *o, int nsteps, double *samples, oracle_frame_cb cb, void *user) {
     try {

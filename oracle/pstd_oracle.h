@@ -32,6 +32,10 @@ void oracle_window(int w, int patch_error, int precision, double *out);
  * outputs (when non-NULL) have L entries; complex arrays interleaved re,im. */
 int oracle_discretization(double dx, int n, int precision, double *wave_numbers, double *complex_factors,
                           double *pressure_factors, double *velocity_factors);
+/* Receiver::get_fft_factors (kernel/core/Receiver.cpp:57-73) over the
+ * discretization of n (table length L = 2^match_number(n)): L complex factors
+ * exp(offset dx k c), interleaved re/im; returns L. */
+int oracle_shift_factors(double dx, int n, int precision, double offset, double *out);
 /* spatderp3 on explicit matrices (kernel_functions.cpp:123-318).  Matrices are
  * row-major rows x cols.  derfact is interleaved complex with derfact_len
  * entries; rho4x2 is the 4x2 pressure or velocity stencil row-major;
@@ -66,6 +70,9 @@ long oracle_read_pml(const oracle_scene *s, int index, int which, double *out);
 int oracle_neighbours(const oracle_scene *s, int index, int dir, int32_t *ids, int max);
 /* Receiver container domain index and row-major p0 index (-1 when outside). */
 int oracle_receiver_cell(const oracle_scene *s, int r, int32_t *domain, int32_t *index);
+/* Receiver r's value on the current state (nearest cell, or spectral
+ * interpolation in PSTD_RECEIVER_SPECTRAL mode) without stepping; 0 or -1. */
+int oracle_receiver_value(oracle_scene *s, int r, double *out);
 /* Advances nsteps time steps from the current frame.  samples: nsteps x
  * n_receivers values (may be NULL).  frames_cb is called for every non-PML
  * domain when frame % SaveNth == 0 (WriteFrame order of kernel/Solver.cpp:80-84). */

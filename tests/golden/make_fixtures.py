@@ -19,10 +19,12 @@
      one domain, 56 with two, 40 with three, 50 with four; 2000 steps, FP64),
      square4096_long_receivers.npz (the 4096 x 4096-cell domain at its
      configured 1000 steps, 16 receivers within 400 cells of the source, FP64;
-     written every 100 steps)
+     written every 100 steps),
+     lshape_spectral_receivers.npz (the L-shape with ten spectral-interpolation
+     receivers, 500 steps, FP64 and FP32)
 
     python tests/golden/make_fixtures.py [--kat] [--series] [--square] [--sources] [--frames] [--square4096]
-                                         [--batch64] [--square4096-long]
+                                         [--batch64] [--square4096-long] [--lshape-spectral] [--sources-more N]
 """
 import json
 import os
@@ -80,6 +82,18 @@ def series(which=("room", "lshape")):
             out[f"fp{prec}"] = o.step(steps)
             print(name, prec, "done", flush=True)
         np.savez(os.path.join(HERE, f"{name}_receivers.npz"), **out)
+
+
+def lshape_spectral(steps=500):
+    """FP64 and FP32 oracle series of the L-shape with spectral-interpolation receivers."""
+    from oracle import binding as ob
+    from openpstd_amd import workloads as W
+    conf = W.lshape_spectral(steps)
+    out = {}
+    for prec in (64, 32):
+        out[f"fp{prec}"] = ob.OracleScene(conf, prec, threads=os.cpu_count() or 1).step(steps)
+        print("lshape_spectral", prec, "done", flush=True)
+    np.savez(os.path.join(HERE, "lshape_spectral_receivers.npz"), **out)
 
 
 def sources(n_sources=2, steps=3000):
@@ -185,5 +199,7 @@ if __name__ == "__main__":
         batch64()
     if "--square4096-long" in args:
         square4096_long()
+    if "--lshape-spectral" in args:
+        lshape_spectral()
     if "--sources-more" in args:
         sources_more(int(args[args.index("--sources-more") + 1]))

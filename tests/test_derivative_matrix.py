@@ -74,3 +74,48 @@ def test_interior_rows_are_toeplitz():
 def test_rejects_inconsistent_shape():
     c = (C.c_double * 4)(0, 0, 0, 0)
     assert lib().pstd_derivative_matrix(100, 128, 0, 0.2, 32, 100, 70.0, c, None, 0) < 0
+
+
+def product_shift_row(F, L, dx, w, n, coef, offset, row, pe=70.0):
+    c = (C.c_double * 4)(*coef)
+    K = lib().pstd_shift_operator_row(F, L, dx, w, n, pe, c, offset, row, None, 0)
+    assert K == n + 2 * w
+    out = np.zeros(K)
+    assert lib().pstd_shift_operator_row(F, L, dx, w, n, pe, c, offset, row,
+                                         out.ctypes.data_as(C.POINTER(C.c_double)), K) == K
+    return out
+
+
+SHIFT_CASES = [
+    # n, w, rho1, rho2, offset, row
+    (100, 32, 1.2, 1.2, 0.37, 40),
+    (77, 32, 2.5, 0.6, 0.9, 0),
+    (77, 32, 2.5, 0.6, 0.05, 76),
+    (50, 16, 1e30, 3.0, 0.5, 3),   # rigid left neighbour, half window
+    (300, 32, 0.9, 1e30, 0.71, 299),
+]
+
+
+@pytest.mark.parametrize("n, w, rho1, rho2, offset, row", SHIFT_CASES)
+def test_shift_row_matches_spatderp3(n, w, rho1, rho2, offset, row):
+    """The spectral receivers' weights: spatderp3 of a pressure line with the
+    shift factors of Receiver::get_fft_factors in place of derfact, one output."""
+    rng = np.random.RandomState(n + row)
+    dx = 0.2
+    F = 2 ** math.ceil(math.log2(n + 2 * w))
+    p1, p2, p3 = rng.randn(3, 40), rng.randn(3, n), rng.randn(3, 45)
+    fac = ob.shift_factors(dx, F, offset)
+    assert len(fac) == F
+    win = ob.window(w, 70, 64)
+    rp, rv = ob.rho_array(rho1, 1.2, rho2, 64)
+    want = ob.spatderp3(p1, p2, p3, fac, rp, rv, win, w, 0, 0, 64)[:, row]
+    r = product_shift_row(F, F, dx, w, n, [rp[2][1], rp[0][0], rp[3][1], rp[1][0]], offset, row)
+    got = np.concatenate([p1[:, -w:], p2, p3[:, :w]], axis=1) @ r
+    assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
+
+
+def test_shift_row_zero_offset_is_the_cell():
+    r = product_shift_row(256, 256, 0.2, 32, 150, [0.0, 0.0, 0.0, 0.0], 0.0, 70)
+    e = np.zeros_like(r)
+    e[32 + 70] = 1.0
+    assert np.abs(r - e).max() < 1e-14

@@ -180,3 +180,67 @@ def test_settings_mirror():
     assert s.GetBandWidth() == pytest.approx(8.67, rel=1e-6)
     room = W.room()
     assert room.num_steps() == 500 and W.lshape().num_steps() == 2000
+
+
+def _spectral_plan(conf):
+    spec = P.PSTDKernel()
+    spec.initialize_kernel(conf)
+    nn_conf = conf
+    nn_conf.receiver_mode = T.PSTD_RECEIVER_PHYSICAL
+    nn = P.PSTDKernel()
+    nn.initialize_kernel(nn_conf)
+    nn_conf.receiver_mode = T.PSTD_RECEIVER_SPECTRAL
+    infos = [spec.domain_info(i) for i in range(spec.domain_count())]
+    return spec.segments(), nn.segments(), infos
+
+
+@pytest.mark.parametrize("walls", ["absorbing", "rigid top", "lshape"])
+def test_spectral_receivers_plan(walls):
+    """Spectral receivers leave the step's segments as they are and add X
+    pressure segments (interpolation only) for exactly the Y-line neighbours
+    beside them that the step does not update along X (rigid walls)."""
+    if walls == "lshape":
+        conf = W.lshape_spectral(10)
+    else:
+        conf = P.PSTDConfiguration()
+        W.base_settings(conf, 10)
+        conf.Domains = [W.domain(0, 0, 100, 60, {"T": 0.0 if walls == "rigid top" else 1.0, "L": 1.0,
+                                                   "B": 1.0, "R": 1.0})]
+        conf.Receivers = [(W._world_pos(40.3), W._world_pos(20.6), 0.0)]
+        conf.receiver_mode = T.PSTD_RECEIVER_SPECTRAL
+    segs, nn_segs, infos = _spectral_plan(conf)
+    assert [s for s in segs if not s["interp_only"]] == nn_segs
+    extra = {s["domain"] for s in segs if s["interp_only"]}
+    for s in segs:
+        if s["interp_only"]:
+            assert s["direction"] == 0 and s["field"] == 0
+    # expected: the sides of each receiver's Y pressure line not updated along X
+    want = set()
+    o = ob.OracleScene(conf, 32)
+    for r in range(len(conf.Receivers)):
+        d, idx = o.receiver_cell(r)
+        col = idx % infos[d]["nx"]
+        (sy,) = [s for s in nn_segs if s["domain"] == d and s["direction"] == 1 and s["field"] == 0
+                 and s["line_begin"] <= col < s["line_end"]]
+        for side in (sy["side1"], sy["side2"]):
+            if side >= 0 and (infos[side]["is_rigid"] or not infos[side]["update_x"]):
+                want.add(side)
+    assert extra == want
+    if walls == "rigid top":
+        assert len(want) == 1
+    if walls == "absorbing":
+        assert not want
+    # the interpolation-only domains' X lines are covered once each
+    for d in extra:
+        cover = np.zeros(infos[d]["ny"], int)
+        for s in segs:
+            if s["interp_only"] and s["domain"] == d:
+                cover[s["line_begin"]:s["line_end"]] += 1
+        assert (cover == 1).all()
+
+
+def test_unknown_receiver_mode():
+    conf = W.room(2)
+    conf.receiver_mode = 7
+    with pytest.raises(P.PSTDError, match="receiver mode"):
+        P.PSTDKernel().initialize_kernel(conf)
