@@ -365,6 +365,53 @@ int pstd_batch_prepare(pstd_kernel *const *kernels, int n, int device, pstd_ctx 
  * the element count (out may be NULL to query it) or a negative status. */
 long long pstd_derivative_matrix(int F, int table_len, int field, double dx, int wlen, int n_main, double patch_error,
                                  const double coef[4], double *out, size_t cap);
+/* ---------------------------------------------------------- .pstd files */
+/* openPSTD's scene/results file (shared/PSTDFile.{h,cpp}): an unqlite 1.1.6
+ * key/value database (read whole into memory, written back whole by
+ * pstd_file_commit) holding the scene as a Boost text archive of
+ * PSTDConfiguration (key {1}), the results scene ({100}), frame counts
+ * ({102, domain}), frames ({103, domain, frame}: float32 p0, row-major) and
+ * receiver series ({104, receiver}: float32, appended per step), version
+ * ({10000} = 4). */
+typedef struct pstd_file pstd_file;
+/* PSTDFile::Open (shared/PSTDFile.cpp:165-178): reads the file, checks the version. */
+int pstd_file_open(const char *path, pstd_file **out);
+/* PSTDFile::New (:180-199): version, the default scene, an empty results scene; written at once. */
+int pstd_file_new(const char *path, pstd_file **out);
+/* PSTDFile::GetSceneConf / GetResultsSceneConf (:206-209, 476-479): the arrays
+ * `out` points at belong to the file until the next call or pstd_file_close;
+ * receiver_mode is PSTD_RECEIVER_PHYSICAL (the file has none); an archive
+ * with other than 6 RK coefficients (an empty scene has none) fills the
+ * first ones and zeros. */
+int pstd_file_get_scene(pstd_file *f, int results, pstd_config *out);
+/* PSTDFile::SetSceneConf (:211-214, 433-452). */
+int pstd_file_set_scene(pstd_file *f, const pstd_config *conf);
+/* PSTDFile::GetResultsFrameCount / GetResultsFrame / GetReceiverData
+ * (:221-231, 487-492): counts, or element counts (out may be NULL to query). */
+int pstd_file_frame_count(pstd_file *f, int domain);
+long long pstd_file_get_frame(pstd_file *f, int frame, int domain, float *out, size_t cap);
+long long pstd_file_get_receiver(pstd_file *f, int receiver, float *out, size_t cap);
+/* Raw records (PSTDFile::GetRawValue / SetRawValue / AppendRawValue /
+ * DeleteValue, shared/PSTDFile.cpp:321-342, 368-402, 456-466); keys are
+ * native 32-bit integers {prefix, ...} (PSTDFile::CreateKey, :344-366).
+ * get_raw returns the value's size (out may be NULL); set_raw with append
+ * extends the value.  record_count / record_key list the keys in byte order
+ * (PSTDFile::OutputDebugInfo, :261-300). */
+long long pstd_file_get_raw(pstd_file *f, const void *key, int nkey, void *out, size_t cap);
+int pstd_file_set_raw(pstd_file *f, const void *key, int nkey, const void *data, long long n, int append);
+int pstd_file_delete_raw(pstd_file *f, const void *key, int nkey);
+int pstd_file_record_count(pstd_file *f);
+int pstd_file_record_key(pstd_file *f, int i, void *out, int cap);
+/* PSTDFile::Commit (:494-501): writes the whole database to the file. */
+int pstd_file_commit(pstd_file *f);
+/* The CLI's `run` (RunCommand::execute, CLI/main.cpp:422-452): delete old
+ * results, initialise new ones, run PSTDKernel(gpu, mcpu) on the file's scene
+ * with frames and receiver samples written into the file (CLIOutput,
+ * CLI/output.cpp:30-39), commit.  Log lines go to log->log. */
+int pstd_file_run(pstd_file *f, int gpu, int mcpu, int receiver_mode, const pstd_callbacks *log);
+const char *pstd_file_last_error(const pstd_file *f);
+void pstd_file_close(pstd_file *f);
+
 /* One row of the spectral receivers' shift operator: the K = n_main + 2w
  * weights of [side1 tail | main | side2 head] whose dot product with a
  * pressure line is spatderp3 (kernel/core/kernel_functions.cpp:123-318) of

@@ -9,7 +9,9 @@ from .config import (PSTDSettings, DomainConfEdge, DomainConf, PSTDConfiguration
 from .kernel import (PSTDKernel, KernelInterface, KernelCallback, KernelCallbackLog, SimulationMetadata,
                      PSTDKernelNotConfiguredException, DeviceSolver, BatchSolver, RecordingCallback)
 from ._lib import PSTDError
+from .pstd_file import PSTDFile
 
 __all__ = ["PSTDSettings", "DomainConfEdge", "DomainConf", "PSTDConfiguration", "RK_COEFFICIENTS", "PSTDKernel",
            "KernelInterface", "KernelCallback", "KernelCallbackLog", "SimulationMetadata",
-           "PSTDKernelNotConfiguredException", "DeviceSolver", "BatchSolver", "RecordingCallback", "PSTDError"]
+           "PSTDKernelNotConfiguredException", "DeviceSolver", "BatchSolver", "RecordingCallback", "PSTDError",
+           "PSTDFile"]

@@ -22,6 +22,10 @@ EXPORTED = [
     "pstd_ctx_last_error", "pstd_ctx_destroy", "pstd_kernel_create", "pstd_kernel_initialize", "pstd_kernel_run",
     "pstd_kernel_get_metadata", "pstd_kernel_domain_info", "pstd_kernel_read_field", "pstd_kernel_read_pml",
     "pstd_kernel_prepare", "pstd_kernel_segment_count", "pstd_kernel_get_segment", "pstd_kernel_last_error", "pstd_kernel_destroy",
+    "pstd_file_open", "pstd_file_new", "pstd_file_get_scene", "pstd_file_set_scene", "pstd_file_frame_count",
+    "pstd_file_get_frame", "pstd_file_get_receiver", "pstd_file_commit", "pstd_file_run", "pstd_file_last_error",
+    "pstd_file_close", "pstd_file_get_raw", "pstd_file_set_raw", "pstd_file_delete_raw", "pstd_file_record_count",
+    "pstd_file_record_key",
 ]
 
 
@@ -72,6 +76,27 @@ def lib():
                                           C.POINTER(C.c_double), C.c_double, C.c_int, C.POINTER(C.c_double),
                                           C.c_size_t]
     L.pstd_shift_operator_row.restype = C.c_longlong
+    fp = C.POINTER(C.c_float)
+    L.pstd_file_open.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.pstd_file_new.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.pstd_file_get_scene.argtypes = [vp, C.c_int, C.POINTER(T.pstd_config)]
+    L.pstd_file_set_scene.argtypes = [vp, C.POINTER(T.pstd_config)]
+    L.pstd_file_frame_count.argtypes = [vp, C.c_int]
+    L.pstd_file_get_frame.argtypes = [vp, C.c_int, C.c_int, fp, C.c_size_t]
+    L.pstd_file_get_frame.restype = C.c_longlong
+    L.pstd_file_get_receiver.argtypes = [vp, C.c_int, fp, C.c_size_t]
+    L.pstd_file_get_receiver.restype = C.c_longlong
+    L.pstd_file_commit.argtypes = [vp]
+    L.pstd_file_run.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(T.pstd_callbacks)]
+    L.pstd_file_last_error.argtypes = [vp]
+    L.pstd_file_last_error.restype = C.c_char_p
+    L.pstd_file_close.argtypes = [vp]
+    L.pstd_file_get_raw.argtypes = [vp, C.c_char_p, C.c_int, C.c_void_p, C.c_size_t]
+    L.pstd_file_get_raw.restype = C.c_longlong
+    L.pstd_file_set_raw.argtypes = [vp, C.c_char_p, C.c_int, C.c_char_p, C.c_longlong, C.c_int]
+    L.pstd_file_delete_raw.argtypes = [vp, C.c_char_p, C.c_int]
+    L.pstd_file_record_count.argtypes = [vp]
+    L.pstd_file_record_key.argtypes = [vp, C.c_int, C.c_void_p, C.c_int]
     L.pstd_ctx_set_form.argtypes = [vp, C.c_int]
     L.pstd_ctx_gemm_work.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.pstd_kernel_set_form.argtypes = [vp, C.c_int]

@@ -18,8 +18,10 @@ LIB = os.path.join(LIBDIR, "libopenpstd_amd.so")
 ARCH = os.environ.get("OPENPSTD_AMD_ARCH", "gfx950")
 
 HIP_SOURCES = ["device/pstd_device.hip"]
-CXX_SOURCES = ["host/scene_model.cpp", "host/pstd_kernel.cpp", "host/capi.cpp", "host/plan_tables.cpp"]
-HEADERS = ["device/fft_line.h", "host/kernel_api.h", "host/scene_model.h", "host/pstd_kernel.h", "host/plan_tables.h"]
+CXX_SOURCES = ["host/scene_model.cpp", "host/pstd_kernel.cpp", "host/capi.cpp", "host/plan_tables.cpp",
+               "host/pstd_file.cpp"]
+HEADERS = ["device/fft_line.h", "host/kernel_api.h", "host/scene_model.h", "host/pstd_kernel.h", "host/plan_tables.h",
+           "host/pstd_file.h"]
 
 # Packed FP32 instructions are kept out of the device code.  No implicit FMA
 # contraction: the device code writes its fused multiply-adds explicitly, so

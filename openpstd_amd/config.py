@@ -88,6 +88,17 @@ class PSTDSettings:
         tmp = int(cround((pe * 0.7 - 17) / 2.0))
         return int(cround(tmp * 2))
 
+    @staticmethod
+    def from_c(s):
+        out = PSTDSettings()
+        out.calctime, out.c1, out.ampMax, out.freqMax = s.calctime, s.c1, s.ampmax, s.freqmax
+        out.rho, out.patcherror, out.tfactRK, out.gridSpacing = s.rho, s.patcherror, s.tfactRK, s.grid_spacing
+        out.band_width, out.wave_length = s.band_width, s.wave_length
+        out.rk_coefficients = [s.rk_coefficients[i] for i in range(6)]
+        out.spectral_interpolation = bool(s.spectral_interpolation)
+        out.PMLCells, out.SaveNth = s.pml_cells, s.save_nth
+        return out
+
     def to_c(self):
         s = T.pstd_settings()
         s.calctime = self.calctime
@@ -224,6 +235,23 @@ class PSTDConfiguration:
     def num_steps(self):
         """Solver::number_of_time_steps = (int)(calctime / dt) in float (kernel/Solver.cpp:41)."""
         return int(_f32(_f32(self.Settings.GetRenderTime()) / self.Settings.GetTimeStep()))
+
+    @staticmethod
+    def from_c(c):
+        """The configuration a pstd_config describes (arrays copied)."""
+        conf = PSTDConfiguration()
+        conf.Settings = PSTDSettings.from_c(c.settings)
+        conf.Speakers = [tuple(c.speakers[3 * i + j] for j in range(3)) for i in range(c.n_speakers)]
+        conf.Receivers = [tuple(c.receivers[3 * i + j] for j in range(3)) for i in range(c.n_receivers)]
+        for i in range(c.n_domains):
+            cd = c.domains[i]
+            d = DomainConf((cd.top_left[0], cd.top_left[1]), (cd.size[0], cd.size[1]))
+            for name in "TLBR":
+                e, ce = getattr(d, name), getattr(cd, name)
+                e.Absorption, e.LR = ce.absorption, bool(ce.lr)
+            conf.Domains.append(d)
+        conf.receiver_mode = c.receiver_mode
+        return conf
 
     def to_c(self):
         """Returns (pstd_config, keepalive) — keep the second item alive while the first is used."""

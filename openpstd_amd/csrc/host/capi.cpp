@@ -1,10 +1,12 @@
 // C ABI of the kernel facade (pstd_kernel_* in include/openpstd_amd.h).
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <string>
 #include <vector>
 
 #include "plan_tables.h"
+#include "pstd_file.h"
 #include "pstd_kernel.h"
 
 using namespace OpenPSTD::Kernel;
@@ -12,6 +14,14 @@ using namespace OpenPSTD::Kernel;
 struct pstd_kernel {
     std::unique_ptr<PSTDKernel> k;
     std::string err;
+};
+
+struct pstd_file {
+    pstdamd::PSTDFileData data;
+    std::string err;
+    // arrays the last pstd_file_get_scene handed out
+    std::vector<float> speakers, receivers;
+    std::vector<pstd_domain_conf> domains;
 };
 
 namespace {
@@ -306,6 +316,195 @@ int pstd_kernel_get_segment(pstd_kernel *k, int i, pstd_plan_segment *out) {
     if (i < 0 || i >= (int)segs.size()) return PSTD_ERR_INVALID;
     *out = segs[i];
     return PSTD_OK;
+}
+
+// ------------------------------------------------------------ .pstd files
+static int file_guarded(pstd_file *f, const std::function<int()> &fn) {
+    if (!f) return PSTD_ERR_INVALID;
+    try {
+        f->err.clear();
+        return fn();
+    } catch (const PSTDSolverError &e) {
+        f->err = e.what();
+        return e.status < 0 ? e.status : PSTD_ERR_INVALID;
+    } catch (const std::exception &e) {
+        f->err = e.what();
+        return PSTD_ERR_INVALID;
+    }
+}
+
+static int file_make(const char *path, bool create, pstd_file **out) {
+    if (!path || !out) return PSTD_ERR_INVALID;
+    *out = new pstd_file();
+    return file_guarded(*out, [&] {
+        if (create) (*out)->data.create(path);
+        else (*out)->data.open(path);
+        return PSTD_OK;
+    });
+}
+
+int pstd_file_open(const char *path, pstd_file **out) { return file_make(path, false, out); }
+
+int pstd_file_new(const char *path, pstd_file **out) { return file_make(path, true, out); }
+
+const char *pstd_file_last_error(const pstd_file *f) { return f ? f->err.c_str() : "null file"; }
+
+void pstd_file_close(pstd_file *f) { delete f; }
+
+int pstd_file_get_scene(pstd_file *f, int results, pstd_config *out) {
+    return file_guarded(f, [&] {
+        if (!out) return PSTD_ERR_INVALID;
+        const pstdamd::SceneConf c =
+            f->data.scene(results ? pstdamd::PSTDFileData::kResultsScene : pstdamd::PSTDFileData::kScene);
+        f->speakers.clear();
+        f->receivers.clear();
+        for (const auto &p : c.speakers) f->speakers.insert(f->speakers.end(), p.begin(), p.end());
+        for (const auto &p : c.receivers) f->receivers.insert(f->receivers.end(), p.begin(), p.end());
+        f->domains = c.domains;
+        *out = pstd_config{};
+        out->settings = c.settings;
+        out->n_speakers = (int32_t)c.speakers.size();
+        out->speakers = f->speakers.data();
+        out->n_receivers = (int32_t)c.receivers.size();
+        out->receivers = f->receivers.data();
+        out->n_domains = (int32_t)c.domains.size();
+        out->domains = f->domains.data();
+        out->receiver_mode = PSTD_RECEIVER_PHYSICAL;
+        return PSTD_OK;
+    });
+}
+
+int pstd_file_set_scene(pstd_file *f, const pstd_config *conf) {
+    return file_guarded(f, [&] {
+        if (!conf) return PSTD_ERR_INVALID;
+        f->data.set_scene(pstdamd::scene_conf_of(*to_conf(conf)));
+        return PSTD_OK;
+    });
+}
+
+int pstd_file_frame_count(pstd_file *f, int domain) {
+    int n = -1;
+    int rc = file_guarded(f, [&] {
+        n = f->data.frame_count((uint32_t)domain);
+        return PSTD_OK;
+    });
+    return rc ? rc : n;
+}
+
+long long pstd_file_get_frame(pstd_file *f, int frame, int domain, float *out, size_t cap) {
+    long long n = -1;
+    int rc = file_guarded(f, [&] {
+        std::vector<float> v = f->data.frame((uint32_t)frame, (uint32_t)domain);
+        n = (long long)v.size();
+        if (out && cap < v.size()) return PSTD_ERR_INVALID;
+        if (out) std::memcpy(out, v.data(), v.size() * sizeof(float));
+        return PSTD_OK;
+    });
+    return rc ? rc : n;
+}
+
+long long pstd_file_get_receiver(pstd_file *f, int receiver, float *out, size_t cap) {
+    long long n = -1;
+    int rc = file_guarded(f, [&] {
+        std::vector<float> v = f->data.receiver((uint32_t)receiver);
+        n = (long long)v.size();
+        if (out && cap < v.size()) return PSTD_ERR_INVALID;
+        if (out) std::memcpy(out, v.data(), v.size() * sizeof(float));
+        return PSTD_OK;
+    });
+    return rc ? rc : n;
+}
+
+long long pstd_file_get_raw(pstd_file *f, const void *key, int nkey, void *out, size_t cap) {
+    long long n = -1;
+    int rc = file_guarded(f, [&] {
+        if (!key || nkey <= 0) return PSTD_ERR_INVALID;
+        auto it = f->data.kv.records.find(std::string((const char *)key, nkey));
+        if (it == f->data.kv.records.end()) {
+            f->err = "no such record";
+            return PSTD_ERR_INVALID;
+        }
+        n = (long long)it->second.size();
+        if (out && cap < it->second.size()) return PSTD_ERR_INVALID;
+        if (out) std::memcpy(out, it->second.data(), it->second.size());
+        return PSTD_OK;
+    });
+    return rc ? rc : n;
+}
+
+int pstd_file_set_raw(pstd_file *f, const void *key, int nkey, const void *data, long long n, int append) {
+    return file_guarded(f, [&] {
+        if (!key || nkey <= 0 || n < 0 || (n > 0 && !data)) return PSTD_ERR_INVALID;
+        std::string &v = f->data.kv.records[std::string((const char *)key, nkey)];
+        if (!append) v.clear();
+        v.append((const char *)data, (size_t)n);
+        return PSTD_OK;
+    });
+}
+
+int pstd_file_delete_raw(pstd_file *f, const void *key, int nkey) {
+    return file_guarded(f, [&] {
+        if (!key || nkey <= 0) return PSTD_ERR_INVALID;
+        return f->data.kv.records.erase(std::string((const char *)key, nkey)) ? PSTD_OK : PSTD_ERR_INVALID;
+    });
+}
+
+int pstd_file_record_count(pstd_file *f) {
+    return f ? (int)f->data.kv.records.size() : PSTD_ERR_INVALID;
+}
+
+int pstd_file_record_key(pstd_file *f, int i, void *out, int cap) {
+    if (!f || i < 0 || i >= (int)f->data.kv.records.size()) return PSTD_ERR_INVALID;
+    auto it = f->data.kv.records.begin();
+    std::advance(it, i);
+    if (out && cap < (int)it->first.size()) return PSTD_ERR_INVALID;
+    if (out) std::memcpy(out, it->first.data(), it->first.size());
+    return (int)it->first.size();
+}
+
+int pstd_file_commit(pstd_file *f) {
+    return file_guarded(f, [&] {
+        f->data.commit();
+        return PSTD_OK;
+    });
+}
+
+namespace {
+// CLIOutput (CLI/output.cpp:30-67): frames and samples into the file, log
+// lines to the caller's callback
+class FileOutput : public CLog {
+public:
+    FileOutput(pstdamd::PSTDFileData &d, const pstd_callbacks *log) : CLog(log), data(d) {}
+    void WriteFrame(int, int domain, PSTD_FRAME_PTR frame) override {
+        data.save_next_frame((uint32_t)domain, frame->data(), frame->size());
+    }
+    void WriteSample(int, int receiver, std::vector<float> v) override {
+        data.append_receiver((uint32_t)receiver, v.data(), v.size());
+    }
+
+private:
+    pstdamd::PSTDFileData &data;
+};
+}  // namespace
+
+int pstd_file_run(pstd_file *f, int gpu, int mcpu, int receiver_mode, const pstd_callbacks *log) {
+    return file_guarded(f, [&] {
+        // RunCommand::execute (CLI/main.cpp:422-452)
+        pstd_config conf;
+        int rc = pstd_file_get_scene(f, 0, &conf);
+        if (rc) return rc;
+        conf.receiver_mode = receiver_mode;
+        auto out = std::make_shared<FileOutput>(f->data, log);
+        out->Info("Delete old results(if any)");
+        f->data.delete_results();
+        out->Info("initilize new results");
+        f->data.initialize_results();
+        PSTDKernel kernel(gpu != 0, mcpu != 0);
+        kernel.initialize_kernel(to_conf(&conf), out);
+        kernel.run(out);
+        f->data.commit();
+        return PSTD_OK;
+    });
 }
 
 }  // extern "C"

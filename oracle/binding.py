@@ -17,10 +17,17 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 
+REF_ROOT = "/root/reference"
+UNQLITE_REF = os.path.join(_HERE, "_ref", "libunqlite_ref.so")
+
+
 def build():
-    """Compiles the oracle with its Makefile (g++ only)."""
+    """Compiles the oracle with its Makefile (g++ only), and openPSTD's own
+    unqlite into _ref/ where the reference checkout exists (ref.mk)."""
     import subprocess
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    if os.path.exists(os.path.join(REF_ROOT, "unqlite", "unqlite.c")):
+        subprocess.run(["make", "-s", "-C", _HERE, "-f", "ref.mk", "REF=" + REF_ROOT], check=True)
 
 
 def lib():
