@@ -15,7 +15,9 @@ weights built in double from each line's segment.  Checked against:
 * graph replay and eager launches: bitwise equal;
 * the GEMM form's state: the same spectral receivers of the 1024 x 1024
   domain in the FFT and the FP16 GEMM form agree to the forms' own
-  difference (rel-L2 <= 1e-4).
+  difference (rel-L2 <= 1e-4);
+* batching: two scenes with spectral receivers advanced together give the
+  bits each gives alone.
 """
 import os
 
@@ -112,3 +114,18 @@ def test_spectral_receivers_in_gemm_form():
     print("GEMM vs FFT form, spectral receivers: rel-L2", e)
     assert np.abs(b).max() > 1e-2
     assert e <= 1e-4
+
+
+def test_spectral_receivers_in_a_batch_are_bitwise_alone():
+    """Two scenes with spectral receivers advanced as one batch (one launch per
+    RK stage, the interpolation passes of both) give the bits each gives alone."""
+    steps = 90
+    a = W.lshape_spectral(steps)
+    b = W.lshape_spectral(steps)
+    b.Speakers = [(W._world_pos(300.5), W._world_pos(60.5), 0.0)]
+    batch = P.BatchSolver([a, b])
+    both = batch.advance(steps)
+    for i, conf in enumerate((a, b)):
+        _, dev = device(conf)
+        alone = dev.advance(steps)
+        assert np.array_equal(batch.scene_samples(both, i), alone), i
