@@ -255,7 +255,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default depends on the workload)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default depends on the workload)")
     ap.add_argument("--workload", default="sources256", choices=WORKLOADS)
-    ap.add_argument("--form", default="gemm_f16", choices=["fft", "gemm_f16", "gemm_bf16"],
+    ap.add_argument("--form", default="gemm_f16", choices=["fft", "gemm_f16", "gemm_bf16", "auto"],
                     help="line derivative form: FP32 FFT, or dense matrices on MFMA (FP16 / BF16 hi+lo split)")
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="replay blocks of this many steps as a hipGraph (default 20; kernel-class timing runs "
@@ -289,6 +289,7 @@ def main():
     confs, desc = workload(args.workload, warmup + 5 * steps + 4)
     phase(f"{args.workload}: {len(confs)} scenes, form {args.form}: building plans and autotuning")
     dev, kernels = make_solver(confs, args.form)
+    form = dev.form()  # with --form auto: the form the upload found faster
     dev.set_graph_steps(20 if args.graph_steps is None else args.graph_steps)
     if args.overlap:
         dev.set_stage_overlap(True)
@@ -316,7 +317,7 @@ def main():
     prof = dev.profile_kernels(steps)
     stage_ms = prof["total"][0] / prof["total"][1]
 
-    if args.form == "fft":
+    if form == "fft":
         cls = max(("fft_small", "fft_4096", "fft_8192"), key=lambda c: prof[c][0])
         mean_ms = prof[cls][0] / prof[cls][1]
         kname = {"fft_small": "stage_small_kernel", "fft_4096": "stage_big_kernel_4096",
@@ -366,10 +367,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": FORM_DTYPE[args.form],
+        "dtype": FORM_DTYPE[form],
         "data": "synthetic",
         "config": {"workload": desc, "scenes": len(confs), "cells_per_stage": cells, "user_cells": user,
-                   "domains": len(infos), "rk_stages_per_step": 6, "form": args.form,
+                   "domains": len(infos), "rk_stages_per_step": 6, "form": form,
+                   "form_requested": args.form,
                    "launch_shape": {"flavour": ["early", "late", "mid"][tuning["lean"]],
                                     "xcd_order": tuning["xcd_order"], "tile_threads": tuning["tile_threads"],
                                     "autotuned": args.tile_order == "auto" and args.flavour == "auto"
@@ -377,7 +379,7 @@ def main():
                                     "autotune_trial_steps": tuning["trial_steps"],
                                     "autotune_cached": tuning["cached"],
                                     "graph_steps": 20 if args.graph_steps is None else args.graph_steps},
-                   "path": ("FP32 FFT (fused LDS line derivative + RK update)" if args.form == "fft" else
+                   "path": ("FP32 FFT (fused LDS line derivative + RK update)" if form == "fft" else
                             "dense derivative matrices on MFMA, 16-bit hi+lo split operands staged from FP32 "
                             "in the kernel, FP32 accumulate, fused RK update (thin PML strips in the FP32 FFT "
                             "form)")},
@@ -397,7 +399,7 @@ def main():
         out["vs_plain"] = value / plain["value"]
     if args.workload == "sources256" and not args.no_accuracy:
         phase("accuracy: 3000 steps vs the FP64 fixture")
-        out["accuracy"] = accuracy(args.form)
+        out["accuracy"] = accuracy(form)
     if not args.no_secondary and args.workload != "lshape":
         phase("secondary: L-shape, FFT form")
         out["secondary"] = secondary_lshape()

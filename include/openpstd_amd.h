@@ -195,7 +195,14 @@ int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan);
 #define PSTD_FORM_FFT 0
 #define PSTD_FORM_GEMM_F16 1
 #define PSTD_FORM_GEMM_BF16 2
+/* PSTD_FORM_AUTO (no counterpart in openPSTD): the plan is uploaded and
+ * autotuned in the FFT form and in the FP16 GEMM form, and the form with the
+ * shorter RK stage is kept (per device and plan signature, so a later upload
+ * of a plan of the same geometry uploads once); pstd_ctx_get_form reports it. */
+#define PSTD_FORM_AUTO 3
 int pstd_ctx_set_form(pstd_ctx *ctx, int form);
+/* The form the uploaded plan runs in (PSTD_FORM_FFT / _GEMM_F16 / _GEMM_BF16). */
+int pstd_ctx_get_form(const pstd_ctx *ctx);
 /* Uploads several independent scenes as one plan: every RK stage of all of
  * them is one launch.  Domains and receivers are numbered across the scenes in
  * order (scene 0's domains first); segments name domains of their own scene.

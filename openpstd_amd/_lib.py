@@ -18,7 +18,7 @@ EXPORTED = [
     "pstd_ctx_set_stage_overlap",
     "pstd_ctx_set_tile_order", "pstd_ctx_set_kernel_flavour", "pstd_ctx_set_tile_threads",
     "pstd_ctx_get_tile_threads", "pstd_ctx_get_tuning", "pstd_ctx_get_autotune", "pstd_batch_prepare",
-    "pstd_derivative_matrix", "pstd_shift_operator_row", "pstd_ctx_set_form", "pstd_kernel_set_form", "pstd_ctx_gemm_work",
+    "pstd_derivative_matrix", "pstd_shift_operator_row", "pstd_ctx_set_form", "pstd_ctx_get_form", "pstd_kernel_set_form", "pstd_ctx_gemm_work",
     "pstd_ctx_last_error", "pstd_ctx_destroy", "pstd_kernel_create", "pstd_kernel_initialize", "pstd_kernel_run",
     "pstd_kernel_get_metadata", "pstd_kernel_domain_info", "pstd_kernel_read_field", "pstd_kernel_read_pml",
     "pstd_kernel_prepare", "pstd_kernel_segment_count", "pstd_kernel_get_segment", "pstd_kernel_last_error", "pstd_kernel_destroy",
@@ -98,6 +98,7 @@ def lib():
     L.pstd_file_record_count.argtypes = [vp]
     L.pstd_file_record_key.argtypes = [vp, C.c_int, C.c_void_p, C.c_int]
     L.pstd_ctx_set_form.argtypes = [vp, C.c_int]
+    L.pstd_ctx_get_form.argtypes = [vp]
     L.pstd_ctx_gemm_work.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.pstd_kernel_set_form.argtypes = [vp, C.c_int]
     L.pstd_ctx_cells_per_stage.argtypes = [vp]

@@ -21,8 +21,9 @@ PSTD_RECEIVER_SPECTRAL = 2
 PSTD_FORM_FFT = 0
 PSTD_FORM_GEMM_F16 = 1
 PSTD_FORM_GEMM_BF16 = 2
+PSTD_FORM_AUTO = 3
 FORMS = {"fft": PSTD_FORM_FFT, "gemm": PSTD_FORM_GEMM_F16, "gemm_f16": PSTD_FORM_GEMM_F16,
-         "gemm_bf16": PSTD_FORM_GEMM_BF16}
+         "gemm_bf16": PSTD_FORM_GEMM_BF16, "auto": PSTD_FORM_AUTO}
 
 
 class pstd_settings(C.Structure):

@@ -1709,7 +1709,8 @@ struct pstd_ctx {
     std::vector<GemmTile> tiles_gemm;
     int n_scenes = 1;
     std::vector<MaxRange> max_ranges;  // state read by a first stage after host writes (GEMM form)
-    int form = PSTD_FORM_FFT;
+    int form = PSTD_FORM_FFT;      // the form the uploaded plan runs in
+    int form_req = PSTD_FORM_FFT;  // as requested (PSTD_FORM_AUTO: chosen at upload)
     double gemm_flops = 0;   // algorithmic MFMA flops of one stage (3 split products)
     long long gemm_cells = 0;
     size_t arena_floats = 0;
@@ -2141,6 +2142,8 @@ struct TuneChoice {
 };
 std::mutex g_tune_mu;
 std::map<std::pair<int, uint64_t>, TuneChoice> g_tune;
+// PSTD_FORM_AUTO's choice per (device, signature of the plan's FFT-form upload)
+std::map<std::pair<int, uint64_t>, int> g_form;
 
 // FNV-1a over what the launches' timing depends on: the form, the scene count,
 // every tile's line class, line count and direction, and the arena size.  Two
@@ -2892,14 +2895,58 @@ static int upload_impl(pstd_ctx *ctx, const pstd_plan_desc *plan, const std::vec
     return rc;
 }
 
+// PSTD_FORM_AUTO: the plan uploaded (and autotuned) in the FFT form and in the
+// FP16 GEMM form, the one with the shorter tuned stage kept; the choice is
+// remembered per (device, plan signature), so a later upload of a plan of
+// the same geometry uploads once.
+static int upload_form(pstd_ctx *ctx, const pstd_plan_desc *plan, const std::vector<int> *dom_scene) {
+    if (ctx->form_req != PSTD_FORM_AUTO) {
+        ctx->form = ctx->form_req;
+        return upload_impl(ctx, plan, dom_scene);
+    }
+    ctx->form = PSTD_FORM_FFT;
+    int rc = upload_impl(ctx, plan, dom_scene);
+    if (rc) return rc;
+    const std::pair<int, uint64_t> key(ctx->device, plan_signature(ctx));
+    int chosen = -1;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_form.find(key);
+        if (it != g_form.end()) chosen = it->second;
+    }
+    if (chosen < 0) {
+        const float fft_ms = ctx->tuned_stage_ms;
+        const int trials = ctx->trial_steps;
+        ctx->form = PSTD_FORM_GEMM_F16;
+        rc = upload_impl(ctx, plan, dom_scene);
+        if (rc) return rc;
+        const bool gemm_better = !ctx->tiles_gemm.empty() && ctx->tuned_stage_ms < fft_ms;
+        chosen = gemm_better ? PSTD_FORM_GEMM_F16 : PSTD_FORM_FFT;
+        ctx->trial_steps += trials;
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mu);
+            g_form[key] = chosen;
+        }
+        if (gemm_better) return PSTD_OK;
+        const int t = ctx->trial_steps;
+        ctx->form = PSTD_FORM_FFT;
+        rc = upload_impl(ctx, plan, dom_scene);  // its launch shape comes from the autotune cache
+        ctx->trial_steps += t;
+        return rc;
+    }
+    if (chosen == PSTD_FORM_FFT) return PSTD_OK;
+    ctx->form = chosen;
+    return upload_impl(ctx, plan, dom_scene);
+}
+
 int pstd_ctx_upload_plan(pstd_ctx *ctx, const pstd_plan_desc *plan) {
     if (!ctx || !plan) return PSTD_ERR_INVALID;
-    return upload_impl(ctx, plan);
+    return upload_form(ctx, plan, nullptr);
 }
 
 int pstd_ctx_upload_plans(pstd_ctx *ctx, const pstd_plan_desc *plans, int n_plans) {
     if (!ctx || !plans || n_plans <= 0) return PSTD_ERR_INVALID;
-    if (n_plans == 1) return upload_impl(ctx, plans);
+    if (n_plans == 1) return upload_form(ctx, plans, nullptr);
     // independent scenes: one plan with the domain lists concatenated; scenes
     // share the time step, the RK coefficients and the grid (one launch per
     // stage advances all of them)
@@ -2952,7 +2999,7 @@ int pstd_ctx_upload_plans(pstd_ctx *ctx, const pstd_plan_desc *plans, int n_plan
     m.segments = segs.data();
     m.n_receivers = (int)rcv.size();
     m.receivers = rcv.data();
-    return upload_impl(ctx, &m, &dom_scene);
+    return upload_form(ctx, &m, &dom_scene);
 }
 
 int pstd_ctx_set_field(pstd_ctx *ctx, int domain, int field, const float *host, size_t count) {
@@ -3152,15 +3199,19 @@ int pstd_ctx_set_graph_steps(pstd_ctx *ctx, int steps) {
 }
 
 int pstd_ctx_set_form(pstd_ctx *ctx, int form) {
-    if (!ctx || (form != PSTD_FORM_FFT && form != PSTD_FORM_GEMM_F16 && form != PSTD_FORM_GEMM_BF16))
+    if (!ctx || (form != PSTD_FORM_FFT && form != PSTD_FORM_GEMM_F16 && form != PSTD_FORM_GEMM_BF16 &&
+                 form != PSTD_FORM_AUTO))
         return PSTD_ERR_INVALID;
     if (ctx->planned) {
         ctx->err = "the derivative form is chosen before the plan is uploaded";
         return PSTD_ERR_INVALID;
     }
-    ctx->form = form;
+    ctx->form_req = form;
+    ctx->form = form == PSTD_FORM_AUTO ? PSTD_FORM_FFT : form;
     return PSTD_OK;
 }
+
+int pstd_ctx_get_form(const pstd_ctx *ctx) { return ctx ? ctx->form : PSTD_ERR_INVALID; }
 
 int pstd_ctx_set_kernel_flavour(pstd_ctx *ctx, int flavour) {
     if (!ctx || flavour < kEarly || flavour > kMid) return PSTD_ERR_INVALID;

@@ -227,7 +227,8 @@ int pstd_kernel_prepare(pstd_kernel *k, pstd_ctx **ctx) {
 }
 
 int pstd_kernel_set_form(pstd_kernel *k, int form) {
-    if (!k || (form != PSTD_FORM_FFT && form != PSTD_FORM_GEMM_F16 && form != PSTD_FORM_GEMM_BF16))
+    if (!k || (form != PSTD_FORM_FFT && form != PSTD_FORM_GEMM_F16 && form != PSTD_FORM_GEMM_BF16 &&
+               form != PSTD_FORM_AUTO))
         return PSTD_ERR_INVALID;
     k->k->set_form(form);
     return PSTD_OK;

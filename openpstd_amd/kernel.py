@@ -130,8 +130,9 @@ class PSTDKernel(KernelInterface):
         return m
 
     def set_form(self, form):
-        """Derivative form of the next run/prepare: "fft" (default) or "gemm"
-        (dense derivative matrices on MFMA, FP16 hi+lo split, FP32 accumulate)."""
+        """Derivative form of the next run/prepare: "fft" (default), "gemm" / "gemm_f16" (dense
+        derivative matrices on MFMA, FP16 hi+lo split, FP32 accumulate), "gemm_bf16", or "auto"
+        (the faster of "fft" and "gemm_f16" for the plan, timed at upload)."""
         _check(lib().pstd_kernel_set_form(self._h, T.FORMS[form]), self._err)
 
     def prepare(self):
@@ -267,6 +268,13 @@ class _Context:
     def set_tile_threads(self, threads):
         """Block size (128, 256 or 512; 128 needs F <= 1024) of the stage kernel for lines of FFT length <= 2048."""
         _check(lib().pstd_ctx_set_tile_threads(self.ctx, int(threads)), self._err)
+
+    def form(self):
+        """The form the uploaded plan runs in ("fft", "gemm_f16", "gemm_bf16"); with set_form("auto")
+        the one the upload found faster."""
+        f = lib().pstd_ctx_get_form(self.ctx)
+        _check(min(f, 0), self._err)
+        return {T.PSTD_FORM_FFT: "fft", T.PSTD_FORM_GEMM_F16: "gemm_f16", T.PSTD_FORM_GEMM_BF16: "gemm_bf16"}[f]
 
     def tuning(self):
         """The autotuner's choice: {"lean" (flavour 0 early, 1 late, 2 mid), "xcd_order", "tile_threads",

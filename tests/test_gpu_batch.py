@@ -227,3 +227,25 @@ def test_batch64_scenes_vs_golden():
             print(f"batch scene {i} ({len(confs[i].Domains)} domains), {form} form, {steps} steps: "
                   f"rel-L2 vs FP64 {e:.2e}, max-abs {m:.2e}")
             assert e <= 1e-4 and m <= 5e-6, (i, form, e, m)
+
+
+@pytest.mark.parametrize("n_scenes", [4, 16])
+def test_auto_form_picks_a_form_and_matches_it_bitwise(n_scenes):
+    """PSTD_FORM_AUTO uploads the plan in the FFT and the FP16 GEMM form and
+    keeps the faster; its results are the bits of that form chosen
+    explicitly, and a second upload of the same plan takes the choice from the
+    cache (fewer trial steps, same form)."""
+    steps = 40
+    confs = W.scene_batch(0, steps, n_scenes=n_scenes)
+    auto = P.BatchSolver(confs, form="auto")
+    chosen = auto.form()
+    assert chosen in ("fft", "gemm_f16")
+    t1 = auto.tuning()["trial_steps"]
+    a = auto.advance(steps)
+    ref = P.BatchSolver(confs, form=chosen)
+    b = ref.advance(steps)
+    print(n_scenes, "scenes: auto chose", chosen, "trial steps", t1, flush=True)
+    assert np.array_equal(a, b)
+    again = P.BatchSolver(confs, form="auto")
+    assert again.form() == chosen
+    assert again.tuning()["trial_steps"] < t1
