@@ -3,12 +3,18 @@
     python -m openpstd_amd create scene.pstd          # a new file with the default scene
     python -m openpstd_amd run -g scene.pstd          # run the scene on the GPU, results into the file
     python -m openpstd_amd info scene.pstd            # scene and result counts
+    python -m openpstd_amd export scene.pstd -F npz   # results as NumPy arrays (HDF5 layout)
 
 `run` follows RunCommand (CLI/main.cpp:365-464): -g selects the GPU solver
 (this library's); without it openPSTD's CPU solvers would run, which this
 library does not provide (an error).  --receivers picks how receivers read
 the pressure (physical: the receiver's cell, reference: openPSTD's
 transposed read, spectral: spectral interpolation at the exact location).
+`export` follows ExportCommand (CLI/main.cpp:476-600) with the layout of
+openPSTD's HDF5 export (shared/export/HDF5Export.cpp:20-80): arrays named
+frame/<domain>/<frame> (dimensions (nx, ny) over the frame's row-major data,
+as openPSTD declares them) and receiver/<receiver>, in one NumPy .npz file
+(<output-dir>/<output-name>.npz): the image has no HDF5 library.
 """
 import argparse
 import sys
@@ -37,6 +43,29 @@ class ConsoleLog(KernelCallbackLog):
             print(message)
 
 
+def export(f, a):
+    """HDF5Export::ExportData's datasets (shared/export/HDF5Export.cpp:20-80)."""
+    import os
+
+    import numpy as np
+
+    from .kernel import PSTDKernel
+    conf = f.GetResultsSceneConf()
+    k = PSTDKernel(GPU=True)
+    k.initialize_kernel(conf)  # the scene's domain sizes (get_metadata; no device work)
+    meta = k.get_metadata().DomainMetadata
+    out = {}
+    domains = a.domains if a.domains else list(range(f.GetResultsDomainCount()))
+    for d in domains:
+        end = a.end_frame if a.end_frame >= 0 else f.GetResultsFrameCount(d) - 1
+        for fr in range(a.start_frame, end + 1):
+            out[f"frame/{d}/{fr}"] = f.GetResultsFrame(fr, d).reshape(meta[d][0], meta[d][1])
+    for r in range(f.GetResultsReceiverCount()):
+        out[f"receiver/{r}"] = f.GetReceiverData(r)
+    os.makedirs(a.output_dir, exist_ok=True)
+    np.savez(os.path.join(a.output_dir, a.output_name + ".npz"), **out)
+
+
 def main(argv=None):
     p = argparse.ArgumentParser(prog="python -m openpstd_amd")
     sub = p.add_subparsers(dest="command", required=True)
@@ -50,6 +79,14 @@ def main(argv=None):
     r.add_argument("--debug", action="store_true")
     i = sub.add_parser("info", help="scene and results of a file")
     i.add_argument("scene_file")
+    e = sub.add_parser("export", help="results as arrays (openPSTD's HDF5 export layout, NumPy .npz)")
+    e.add_argument("scene_file")
+    e.add_argument("-F", "--format", default="npz", choices=["npz"])
+    e.add_argument("-d", "--output-dir", default="./")
+    e.add_argument("-n", "--output-name", default="result")
+    e.add_argument("-s", "--start-frame", type=int, default=0)
+    e.add_argument("-e", "--end-frame", type=int, default=-1)
+    e.add_argument("-D", "--domains", type=int, nargs="*", default=None)
     a = p.parse_args(argv)
     try:
         if a.command == "create":
@@ -58,6 +95,8 @@ def main(argv=None):
             f = PSTDFile.Open(a.scene_file)
             f.run(gpu=a.gpu_accelerated, mcpu=a.multithreaded, receiver_mode=MODES[a.receivers],
                   log=ConsoleLog(a.debug))
+        elif a.command == "export":
+            export(PSTDFile.Open(a.scene_file), a)
         else:
             f = PSTDFile.Open(a.scene_file)
             conf, res = f.GetSceneConf(), f.GetResultsSceneConf()

@@ -357,3 +357,37 @@ def test_cli_create_and_info(tmp_path, capsys):
     assert "2 domains, 1 speakers, 1 receivers" in out
     # without -g openPSTD's CPU solver would run, which this library does not provide
     assert main(["run", path]) == 1
+
+
+def test_cli_export_layout(tmp_path):
+    """`export` writes openPSTD's HDF5 export datasets (frame/<d>/<f> with the
+    dimensions openPSTD declares, (nx, ny), over the row-major frame data;
+    receiver/<r>) as NumPy arrays."""
+    from openpstd_amd.__main__ import main
+    path = str(tmp_path / "exp.pstd")
+    conf = W.lshape(0, 10)
+    f = P.PSTDFile.New(path)
+    f.SetSceneConf(conf)
+    f.SetRawValue(P.PSTDFile.CreateKey(100), f.GetRawValue(P.PSTDFile.CreateKey(1)))
+    rng = np.random.RandomState(1)
+    frames = {}
+    for d in range(3):
+        for fr in range(2):
+            frames[d, fr] = rng.randn(128 * 256).astype(np.float32)
+            f.SetRawValue(P.PSTDFile.CreateKey(103, (d, fr)), frames[d, fr].tobytes())
+        f.SetRawValue(P.PSTDFile.CreateKey(102, (d,)), np.int32(2).tobytes())
+    series = rng.randn(4, 7).astype(np.float32)
+    for r in range(4):
+        f.SetRawValue(P.PSTDFile.CreateKey(104, (r,)), series[r].tobytes())
+    f.Commit()
+    assert main(["export", path, "-d", str(tmp_path), "-n", "out", "-e", "1", "-D", "0", "2"]) == 0
+    z = np.load(tmp_path / "out.npz")
+    assert sorted(z.files) == sorted([f"frame/{d}/{fr}" for d in (0, 2) for fr in (0, 1)] +
+                                     [f"receiver/{r}" for r in range(4)])
+    for d in (0, 2):
+        for fr in (0, 1):
+            a = z[f"frame/{d}/{fr}"]
+            assert a.shape == (256, 128)
+            assert np.array_equal(a.ravel(), frames[d, fr])
+    for r in range(4):
+        assert np.array_equal(z[f"receiver/{r}"], series[r])
