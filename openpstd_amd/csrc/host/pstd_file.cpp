@@ -57,9 +57,11 @@ uint32_t kv_hash(const void *key, size_t n) {
 }
 
 void KVFile::load(const std::string &path) {
-    std::ifstream f(path, std::ios::binary);
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
     if (!f) throw std::runtime_error("cannot open " + path);
-    std::vector<uint8_t> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::vector<uint8_t> buf((size_t)f.tellg());
+    f.seekg(0);
+    if (!f.read((char *)buf.data(), (std::streamsize)buf.size())) throw std::runtime_error("cannot read " + path);
     if (buf.size() < 512 || std::memcmp(buf.data(), kDbSig, 7) != 0 || be(&buf[7], 4) != kDbMagic)
         throw std::runtime_error(path + ": not an unqlite database");
     const uint64_t ps = be(&buf[19], 4);
@@ -112,6 +114,8 @@ void KVFile::load(const std::string &path) {
             // key from the first overflow page on, data from the page and
             // offset that page records, both along the chain
             if (ovfl < 2 || ovfl >= npages || nkey > (1u << 30) || ndata > buf.size()) throw Bad{};
+            key.reserve(nkey);
+            data.reserve(ndata);
             const uint8_t *o = page(ovfl);
             const uint64_t dpage = be(o + 8, 8), doff = be(o + 16, 2);
             if (dpage < 2 || dpage >= npages || doff > ps) throw Bad{};
