@@ -266,6 +266,8 @@ def main():
                     help="FFT stage-kernel flavour (default: the autotuner's choice)")
     ap.add_argument("--tile-threads", default="auto", choices=["auto", "128", "256", "512"],
                     help="block size for lines of FFT length <= 2048 (default: the autotuner's choice)")
+    ap.add_argument("--receivers", default="physical", choices=["physical", "spectral"],
+                    help="receiver reading: the receiver's cell, or spectral interpolation at its location")
     ap.add_argument("--overlap", action="store_true",
                     help="GEMM form: run a stage's FFT-form tiles on a second stream beside the GEMM kernel")
     ap.add_argument("--plain-compare", type=int, default=8, metavar="N",
@@ -287,6 +289,12 @@ def main():
     steps = args.steps if args.steps is not None else DEFAULT_STEPS[args.workload]
     warmup = args.warmup if args.warmup is not None else DEFAULT_WARMUP[args.workload]
     confs, desc = workload(args.workload, warmup + 5 * steps + 4)
+    if args.receivers == "spectral":
+        from openpstd_amd import abi_types as T
+        for c in confs:
+            c.receiver_mode = T.PSTD_RECEIVER_SPECTRAL
+        desc += "; spectral-interpolation receivers"
+        args.no_accuracy = True  # the FP64 fixture holds nearest-cell series
     phase(f"{args.workload}: {len(confs)} scenes, form {args.form}: building plans and autotuning")
     dev, kernels = make_solver(confs, args.form)
     form = dev.form()  # with --form auto: the form the upload found faster
