@@ -402,8 +402,9 @@ long long pstd_file_get_receiver(pstd_file *f, int receiver, float *out, size_t 
  * DeleteValue, shared/PSTDFile.cpp:321-342, 368-402, 456-466); keys are
  * native 32-bit integers {prefix, ...} (PSTDFile::CreateKey, :344-366).
  * get_raw returns the value's size (out may be NULL); set_raw with append
- * extends the value.  record_count / record_key list the keys in byte order
- * (PSTDFile::OutputDebugInfo, :261-300). */
+ * extends the value.  record_count takes a list of the keys in byte order
+ * (PSTDFile::OutputDebugInfo, :261-300); record_key reads key i of that
+ * list. */
 long long pstd_file_get_raw(pstd_file *f, const void *key, int nkey, void *out, size_t cap);
 int pstd_file_set_raw(pstd_file *f, const void *key, int nkey, const void *data, long long n, int append);
 int pstd_file_delete_raw(pstd_file *f, const void *key, int nkey);

@@ -22,6 +22,8 @@ struct pstd_file {
     // arrays the last pstd_file_get_scene handed out
     std::vector<float> speakers, receivers;
     std::vector<pstd_domain_conf> domains;
+    // the keys in byte order, as of the last pstd_file_record_count
+    std::vector<std::string> keys;
 };
 
 namespace {
@@ -451,16 +453,18 @@ int pstd_file_delete_raw(pstd_file *f, const void *key, int nkey) {
 }
 
 int pstd_file_record_count(pstd_file *f) {
-    return f ? (int)f->data.kv.records.size() : PSTD_ERR_INVALID;
+    if (!f) return PSTD_ERR_INVALID;
+    f->keys.clear();
+    for (const auto &kv : f->data.kv.records) f->keys.push_back(kv.first);
+    return (int)f->keys.size();
 }
 
 int pstd_file_record_key(pstd_file *f, int i, void *out, int cap) {
-    if (!f || i < 0 || i >= (int)f->data.kv.records.size()) return PSTD_ERR_INVALID;
-    auto it = f->data.kv.records.begin();
-    std::advance(it, i);
-    if (out && cap < (int)it->first.size()) return PSTD_ERR_INVALID;
-    if (out) std::memcpy(out, it->first.data(), it->first.size());
-    return (int)it->first.size();
+    if (!f || i < 0 || i >= (int)f->keys.size()) return PSTD_ERR_INVALID;
+    const std::string &k = f->keys[i];
+    if (out && cap < (int)k.size()) return PSTD_ERR_INVALID;
+    if (out) std::memcpy(out, k.data(), k.size());
+    return (int)k.size();
 }
 
 int pstd_file_commit(pstd_file *f) {
