@@ -391,3 +391,33 @@ def test_cli_export_layout(tmp_path):
             assert np.array_equal(a.ravel(), frames[d, fr])
     for r in range(4):
         assert np.array_equal(z[f"receiver/{r}"], series[r])
+
+
+def test_damaged_files_raise_and_never_crash(tmp_path):
+    """Bytes of a valid file overwritten at random (header, map, cells,
+    overflow pages): Open either reads the file or raises PSTDError."""
+    src = str(tmp_path / "ok.pstd")
+    want = records(3, 400, big_every=9)
+    f = P.PSTDFile.New(src)
+    for k, v in want.items():
+        f.SetRawValue(k, v)
+    f.Commit()
+    del f
+    raw = bytearray(open(src, "rb").read())
+    rng = np.random.RandomState(0)
+    outcomes = {"read": 0, "error": 0}
+    for trial in range(300):
+        b = bytearray(raw)
+        page = rng.randint(len(b) // 4096)
+        for _ in range(rng.randint(1, 16)):
+            b[page * 4096 + rng.randint(4096)] = rng.randint(256)
+        path = str(tmp_path / f"bad{trial % 4}.pstd")
+        open(path, "wb").write(bytes(b))
+        try:
+            g = P.PSTDFile.Open(path)
+            g.keys()
+            outcomes["read"] += 1
+        except P.PSTDError:
+            outcomes["error"] += 1
+    assert outcomes["read"] + outcomes["error"] == 300
+    print(outcomes)
