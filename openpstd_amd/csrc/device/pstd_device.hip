@@ -966,7 +966,10 @@ __device__ __forceinline__ float wave_sum(float v) {
 __global__ __launch_bounds__(256) void si_line_kernel(const float *__restrict__ arena, const SILine *__restrict__ lines,
                                                       int nlines, const float *__restrict__ coef,
                                                       float *__restrict__ col) {
-    const int lane = threadIdx.x & 63, j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // jobs are sorted by line: the receivers reading one line are neighbours,
+    // and XCD-contiguous runs of blocks put them on one XCD (its L2 serves the
+    // line after the first read)
+    const int lane = threadIdx.x & 63, j = tile_index(blockIdx.x, gridDim.x, 1) * 4 + (threadIdx.x >> 6);
     if (j >= nlines) return;
     const SILine L = lines[j];
     const int w = L.w, n = L.n_main, K = n + 2 * w;
@@ -2515,6 +2518,7 @@ static int build_spectral_receivers(pstd_ctx *ctx, const pstd_plan_desc *plan) {
         col_total += P.K;
     }
     if (points.empty()) return PSTD_OK;
+    std::stable_sort(lines.begin(), lines.end(), [](const SILine &x, const SILine &y) { return x.m_px < y.m_px; });
     HIP_TRY(hipMalloc(&ctx->d_si_lines, lines.size() * sizeof(SILine)));
     HIP_TRY(hipMalloc(&ctx->d_si_points, points.size() * sizeof(SIPoint)));
     HIP_TRY(hipMalloc(&ctx->d_si_coef, coef.size() * sizeof(float)));
