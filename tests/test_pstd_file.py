@@ -25,7 +25,6 @@ import pytest
 
 import openpstd_amd as P
 from openpstd_amd import workloads as W
-from openpstd_amd._lib import lib
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = os.path.join(HERE, "..", "oracle", "_ref", "libunqlite_ref.so")
