@@ -56,3 +56,31 @@ def test_cli_run_writes_the_kernel_results(tmp_path):
             for i, want in enumerate(fl):
                 assert db.fetch(P.PSTDFile.CreateKey(103, (d, i))) == want.tobytes()
         db.close()
+
+
+def test_cli_run_spectral_receivers(tmp_path):
+    """`run -g --receivers spectral`: the receiver records hold the spectral
+    receivers' series, bitwise those of the kernel API in that mode."""
+    from openpstd_amd import abi_types as T
+    steps = 60
+    conf = W.lshape_spectral(steps)
+    conf.Settings.SetSaveNth(1)
+    path = str(tmp_path / "spectral.pstd")
+    f = P.PSTDFile.New(path)
+    f.SetSceneConf(conf)
+    f.Commit()
+    del f
+    from openpstd_amd.__main__ import main
+    assert main(["run", "-g", path, "--receivers", "spectral"]) == 0
+    g = P.PSTDFile.Open(path)
+    scene = g.GetSceneConf()
+    scene.receiver_mode = T.PSTD_RECEIVER_SPECTRAL
+    k = P.PSTDKernel(GPU=True)
+    k.initialize_kernel(scene)
+    rec = P.RecordingCallback()
+    k.run(rec)
+    assert sorted(rec.samples) == list(range(len(conf.Receivers)))
+    for r, series in rec.samples.items():
+        want = np.array([v for _, v in series], np.float32)
+        assert want.size == steps
+        assert np.array_equal(g.GetReceiverData(r), want)
