@@ -7,6 +7,13 @@
  * method in double precision).  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it.  It is never part of the product
  * path.
+ *
+ * Parity unpinned (no golden vector, invariant or closed form fixes them):
+ * the px/py split of the speaker's initial pressure (openPSTD's Gaussian
+ * test fixes only their sum), the REFERENCE receiver mode's read through
+ * openPSTD's column-major storage (checked against hand-computed cells
+ * only), and bitwise agreement with a reference binary (FFTW's and Eigen's
+ * rounding order): no build of openPSTD's kernel exists here.
  */
 #ifndef PSTD_ORACLE_H
 #define PSTD_ORACLE_H
