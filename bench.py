@@ -329,7 +329,7 @@ def main():
         cls = max(("fft_small", "fft_4096", "fft_8192"), key=lambda c: prof[c][0])
         mean_ms = prof[cls][0] / prof[cls][1]
         kname = {"fft_small": "stage_small_kernel", "fft_4096": "stage_big_kernel_4096",
-                 "fft_8192": "stage_big_kernel_8192"}[cls]
+                 "fft_8192": "stage_split_kernel"}[cls]
         # one event pair around as many steps again, replayed like the timed
         # ones (per-launch pairs inflate launches of a few microseconds): the
         # stage time of the HBM roofline
