@@ -399,7 +399,10 @@ long long pstd_file_get_frame(pstd_file *f, int frame, int domain, float *out, s
     int rc = file_guarded(f, [&] {
         std::vector<float> v = f->data.frame((uint32_t)frame, (uint32_t)domain);
         n = (long long)v.size();
-        if (out && cap < v.size()) return PSTD_ERR_INVALID;
+        if (out && cap < v.size()) {
+            f->err = "output buffer smaller than the record";
+            return PSTD_ERR_INVALID;
+        }
         if (out) std::memcpy(out, v.data(), v.size() * sizeof(float));
         return PSTD_OK;
     });
@@ -411,7 +414,10 @@ long long pstd_file_get_receiver(pstd_file *f, int receiver, float *out, size_t 
     int rc = file_guarded(f, [&] {
         std::vector<float> v = f->data.receiver((uint32_t)receiver);
         n = (long long)v.size();
-        if (out && cap < v.size()) return PSTD_ERR_INVALID;
+        if (out && cap < v.size()) {
+            f->err = "output buffer smaller than the record";
+            return PSTD_ERR_INVALID;
+        }
         if (out) std::memcpy(out, v.data(), v.size() * sizeof(float));
         return PSTD_OK;
     });
@@ -428,7 +434,10 @@ long long pstd_file_get_raw(pstd_file *f, const void *key, int nkey, void *out, 
             return PSTD_ERR_INVALID;
         }
         n = (long long)it->second.size();
-        if (out && cap < it->second.size()) return PSTD_ERR_INVALID;
+        if (out && cap < it->second.size()) {
+            f->err = "output buffer smaller than the record";
+            return PSTD_ERR_INVALID;
+        }
         if (out) std::memcpy(out, it->second.data(), it->second.size());
         return PSTD_OK;
     });
@@ -462,7 +471,10 @@ int pstd_file_record_count(pstd_file *f) {
 int pstd_file_record_key(pstd_file *f, int i, void *out, int cap) {
     if (!f || i < 0 || i >= (int)f->keys.size()) return PSTD_ERR_INVALID;
     const std::string &k = f->keys[i];
-    if (out && cap < (int)k.size()) return PSTD_ERR_INVALID;
+    if (out && cap < (int)k.size()) {
+        f->err = "output buffer smaller than the key";
+        return PSTD_ERR_INVALID;
+    }
     if (out) std::memcpy(out, k.data(), k.size());
     return (int)k.size();
 }

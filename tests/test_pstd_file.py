@@ -420,3 +420,22 @@ def test_damaged_files_raise_and_never_crash(tmp_path):
             outcomes["error"] += 1
     assert outcomes["read"] + outcomes["error"] == 300
     print(outcomes)
+
+
+def test_undersized_output_buffers_are_rejected(tmp_path):
+    """The C calls that copy a record out refuse a buffer smaller than the
+    record (nothing is written past `cap`) and say why."""
+    from openpstd_amd import abi_types as T
+    from openpstd_amd._lib import lib
+    f = P.PSTDFile.New(str(tmp_path / "cap.pstd"))
+    key = b"some-key"
+    f.SetRawValue(key, b"0123456789")
+    buf = C.create_string_buffer(b"\xee" * 16, 16)
+    assert lib().pstd_file_get_raw(f._h, key, len(key), buf, 9) == T.PSTD_ERR_INVALID
+    assert "smaller" in f._err()
+    assert buf.raw == b"\xee" * 16
+    assert lib().pstd_file_get_raw(f._h, key, len(key), buf, 10) == 10
+    assert buf.raw[:10] == b"0123456789"
+    i = f.keys().index(key)  # the listing is taken by pstd_file_record_count
+    assert lib().pstd_file_record_key(f._h, i, buf, 3) == T.PSTD_ERR_INVALID
+    assert "smaller" in f._err()
